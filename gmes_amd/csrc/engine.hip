@@ -1,0 +1,996 @@
+// C-ABI of the MI355X FDTD engine (include/gmes_engine.h).
+//
+// Host side of the engine: device memory for one sim, validation of every
+// index a kernel will use (so a bad table becomes an error code, never an
+// out-of-bounds access), grouping of source operations per cell, the
+// per-step launch sequence (gmes/fdtd.py:1099-1122), HIP-graph capture of
+// one step, HIP-event kernel timing and batches of sims.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "engine_kernels.h"
+
+using namespace gmes_engine;
+
+namespace {
+
+thread_local char g_err[512];
+
+int set_error(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return set_error(e_ == hipErrorOutOfMemory ? GMES_ENOMEM : GMES_EDEVICE, "%s: %s", \
+                       #expr, hipGetErrorString(e_));                                   \
+  } while (0)
+
+const int kAx1[6] = {1, 2, 0, 1, 2, 0};
+const int kAx2[6] = {2, 0, 1, 2, 0, 1};
+const int kGhostOrder[6][2] = {{1, 2}, {2, 0}, {0, 1}, {1, 2}, {2, 0}, {0, 1}};
+
+bool active(const gmes_sim* s, int c) { return (s->g.active >> c) & 1u; }
+
+int dev_alloc(gmes_sim* s, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  HIP_TRY(hipMalloc(p, bytes));
+  s->bytes += (int64_t)bytes;
+  return GMES_OK;
+}
+
+void dev_free(gmes_sim* s, void* p, size_t bytes) {
+  if (p) {
+    (void)hipFree(p);
+    s->bytes -= (int64_t)bytes;
+  }
+}
+
+int state_words(const gmes_mat_rec& r) {
+  switch (r.cls) {
+    case GMES_CLS_DRUDE: return 2 * r.n1;
+    case GMES_CLS_DCP_ADE: return 1 + 2 * r.n1 + 2 * r.n2;
+    case GMES_CLS_DCP_PLRC: return r.n1 + 2 * r.n2;
+    case GMES_CLS_DM2: return 3 * r.n1;
+    default: return 0;
+  }
+}
+
+size_t real_size(const gmes_sim* s) { return s->fp64 ? 8 : 4; }
+
+// bytes of one ADE state word of a class
+size_t ade_word(const gmes_sim* s, int cls) {
+  if (cls == GMES_CLS_DCP_PLRC) return 2 * real_size(s);
+  if (cls == GMES_CLS_DM2) return real_size(s);
+  return (size_t)s->esize;
+}
+
+void free_ade(gmes_sim* s) {
+  for (int c = 0; c < 6; ++c)
+    for (int k = 0; k < GMES_NCLS; ++k) {
+      AdeList& L = s->ade[c][k];
+      if (L.cell) (void)hipFree(L.cell);
+      if (L.rec) (void)hipFree(L.rec);
+      if (L.state) (void)hipFree(L.state);
+      L = AdeList();
+    }
+}
+
+template <class R>
+int upload_tables(gmes_sim* s) {
+  std::vector<DevRec<R>> dr(s->recs.size());
+  for (size_t i = 0; i < s->recs.size(); ++i) {
+    const gmes_mat_rec& r = s->recs[i];
+    dr[i].cls = r.cls;
+    dr[i].n1 = r.n1;
+    dr[i].n2 = r.n2;
+    dr[i].prof = r.prof;
+    dr[i].coef_off = r.coef_off;
+    dr[i].c0 = (R)(s->g.dt / r.eps_mu);  // dt / eps_inf in FP64, rounded once
+    dr[i].eps_mu = (R)r.eps_mu;
+    dr[i].val_re = (R)r.value[0];
+    dr[i].val_im = (R)r.value[1];
+  }
+  std::vector<R> cf(s->coef.size());
+  for (size_t i = 0; i < cf.size(); ++i) cf[i] = (R)s->coef[i];
+  // profiles into one buffer with offsets
+  std::vector<int64_t> off(64 * 18, 0);
+  std::vector<R> pr;
+  for (int p = 0; p < s->nprof; ++p)
+    for (int c = 0; c < 6; ++c)
+      for (int a = 0; a < 3; ++a) {
+        off[(p * 6 + c) * 3 + a] = (int64_t)pr.size();
+        for (double v : s->prof_host[p][c][a]) pr.push_back((R)v);
+      }
+  if (pr.empty()) pr.push_back(R(0));
+  if (cf.empty()) cf.push_back(R(0));
+  int rc;
+  if ((rc = dev_alloc(s, &s->d_recs, dr.size() * sizeof(DevRec<R>)))) return rc;
+  if ((rc = dev_alloc(s, &s->d_coef, cf.size() * sizeof(R)))) return rc;
+  if ((rc = dev_alloc(s, &s->d_prof, pr.size() * sizeof(R)))) return rc;
+  if ((rc = dev_alloc(s, (void**)&s->d_prof_off, off.size() * sizeof(int64_t)))) return rc;
+  HIP_TRY(hipMemcpy(s->d_recs, dr.data(), dr.size() * sizeof(DevRec<R>), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s->d_coef, cf.data(), cf.size() * sizeof(R), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s->d_prof, pr.data(), pr.size() * sizeof(R), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s->d_prof_off, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  return GMES_OK;
+}
+
+template <class R>
+int upload_sources(gmes_sim* s) {
+  // stable grouping by (phase, comp, cell); E groups first
+  std::vector<int64_t> order(s->ops.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+    const gmes_src_op &a = s->ops[x], &b = s->ops[y];
+    const int pa = a.comp >= 3, pb = b.comp >= 3;
+    if (pa != pb) return pa < pb;
+    if (a.comp != b.comp) return a.comp < b.comp;
+    return a.cell < b.cell;
+  });
+  std::vector<DevSrcOp<R>> dops(order.size());
+  std::vector<SrcGroup> groups;
+  s->ngroups_e = s->ngroups_h = 0;
+  for (size_t n = 0; n < order.size(); ++n) {
+    const gmes_src_op& o = s->ops[order[n]];
+    DevSrcOp<R>& d = dops[n];
+    d.kind = o.kind; d.wave = o.wave; d.aux = o.aux; d.aux_comp = o.aux_comp;
+    d.i0 = o.i0; d.i1 = o.i1; d.k = (R)o.k; d.r0 = (R)o.r0; d.r1 = (R)o.r1;
+    if (groups.empty() || groups.back().comp != o.comp || groups.back().cell != o.cell) {
+      SrcGroup g;
+      g.comp = o.comp; g.cell = o.cell; g.begin = (int32_t)n; g.end = (int32_t)n + 1; g.pad_ = 0;
+      groups.push_back(g);
+      if (o.comp < 3) s->ngroups_e++; else s->ngroups_h++;
+    } else {
+      groups.back().end = (int32_t)n + 1;
+    }
+  }
+  if (s->d_ops) { dev_free(s, s->d_ops, 0); s->d_ops = nullptr; }
+  if (s->d_groups) { dev_free(s, s->d_groups, 0); s->d_groups = nullptr; }
+  int rc;
+  if ((rc = dev_alloc(s, &s->d_ops, std::max<size_t>(1, dops.size()) * sizeof(DevSrcOp<R>)))) return rc;
+  if ((rc = dev_alloc(s, (void**)&s->d_groups, std::max<size_t>(1, groups.size()) * sizeof(SrcGroup)))) return rc;
+  if (!dops.empty())
+    HIP_TRY(hipMemcpy(s->d_ops, dops.data(), dops.size() * sizeof(DevSrcOp<R>), hipMemcpyHostToDevice));
+  if (!groups.empty())
+    HIP_TRY(hipMemcpy(s->d_groups, groups.data(), groups.size() * sizeof(SrcGroup), hipMemcpyHostToDevice));
+  return GMES_OK;
+}
+
+// ------------------------------------------------------------ launching
+template <class T>
+FieldArgs<T> field_args(const gmes_sim* s) {
+  FieldArgs<T> a;
+  for (int c = 0; c < 6; ++c) {
+    a.f[c] = static_cast<T*>(s->f[c]);
+    for (int d = 0; d < 3; ++d) a.s[c][d] = (int)s->shape[c][d];
+  }
+  a.active = s->g.active;
+  return a;
+}
+
+int next_pow2(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+void time_begin(gmes_sim* s, int fam, hipStream_t st) {
+  if (!s->timing.on) return;
+  hipEvent_t a, b;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  (void)hipEventRecord(a, st);
+  s->timing.pending.push_back({fam, {a, b}});
+}
+
+void time_end(gmes_sim* s, hipStream_t st) {
+  if (!s->timing.on) return;
+  (void)hipEventRecord(s->timing.pending.back().second.second, st);
+}
+
+template <class T>
+void enqueue_ghost(gmes_sim* s, bool electric, int tick, hipStream_t st) {
+  using R = typename real_of<T>::type;
+  GhostArgs<T> g{};
+  g.fa = field_args<T>(s);
+  g.ncomp = 0;
+  int64_t total = 0;
+  for (int c = electric ? 0 : 3; c < (electric ? 3 : 6); ++c) {
+    if (!active(s, c)) continue;
+    const int t = g.ncomp++;
+    const int a1 = kGhostOrder[c][0], a2 = kGhostOrder[c][1], a0 = c % 3;
+    g.comp[t] = c;
+    g.ax[t][0] = a1; g.ax[t][1] = a2; g.ax[t][2] = a0;
+    g.count[t] = s->shape[c][a0] * s->shape[c][a2] + s->shape[c][a0] * (s->shape[c][a1] - 1);
+    g.ph[t][0] = cplx<R>{(R)s->g.phase[c][a1][0], (R)s->g.phase[c][a1][1]};
+    g.ph[t][1] = cplx<R>{(R)s->g.phase[c][a2][0], (R)s->g.phase[c][a2][1]};
+    total += g.count[t];
+  }
+  g.tick = tick;
+  g.ctl = reinterpret_cast<Ctl*>(s->d_clock);
+  time_begin(s, 4, st);
+  launch_ghost<T>(g, total, st);
+  time_end(s, st);
+}
+
+template <class T>
+void enqueue_main(gmes_sim* s, bool electric, hipStream_t st) {
+  using R = typename real_of<T>::type;
+  MainArgs<T> m{};
+  m.fa = field_args<T>(s);
+  int ext[3] = {0, 0, 0};
+  bool any = false;
+  for (int c = electric ? 0 : 3; c < (electric ? 3 : 6); ++c) {
+    if (!active(s, c)) continue;
+    any = true;
+    for (int d = 0; d < 3; ++d) ext[d] = std::max<int>(ext[d], (int)s->shape[c][d]);
+  }
+  if (!any) return;
+  for (int c = 0; c < 6; ++c) {
+    m.ids[c] = s->ids[c];
+    m.pml[c] = s->pml_state[c];
+    m.ncell[c] = (int)s->ncell[c];
+  }
+  m.recs = static_cast<const DevRec<R>*>(s->d_recs);
+  m.prof = static_cast<const R*>(s->d_prof);
+  m.prof_off = s->d_prof_off;
+  m.d[0] = (R)s->g.dx; m.d[1] = (R)s->g.dy; m.d[2] = (R)s->g.dz;
+  int bx = s->tile_bx, by = s->tile_by;
+  if (bx <= 0 || by <= 0) {
+    bx = std::min(64, next_pow2(ext[2]));
+    by = std::max(1, std::min(256 / bx, next_pow2(ext[1])));
+  }
+  time_begin(s, electric ? 0 : 1, st);
+  launch_main<T>(electric, m, ext, bx, by, st);
+  time_end(s, st);
+}
+
+template <class T>
+void enqueue_ade(gmes_sim* s, hipStream_t st) {
+  using R = typename real_of<T>::type;
+  for (int c = 0; c < 3; ++c)
+    for (int cls = GMES_CLS_DRUDE; cls < GMES_NCLS; ++cls) {
+      const AdeList& L = s->ade[c][cls];
+      if (L.n == 0) continue;
+      AdeArgs<T> a{};
+      a.fa = field_args<T>(s);
+      a.cell = L.cell; a.rec = L.rec; a.state = L.state; a.n = (int)L.n;
+      a.recs = static_cast<const DevRec<R>*>(s->d_recs);
+      a.coef = static_cast<const R*>(s->d_coef);
+      a.d[0] = (R)s->g.dx; a.d[1] = (R)s->g.dy; a.d[2] = (R)s->g.dz;
+      a.dt = (R)s->g.dt;
+      a.ctl = reinterpret_cast<Ctl*>(s->d_clock);
+      time_begin(s, 2, st);
+      launch_ade<T>(cls, c, a, st);
+      time_end(s, st);
+    }
+}
+
+template <class T>
+void enqueue_src(gmes_sim* s, bool electric, hipStream_t st) {
+  using R = typename real_of<T>::type;
+  const int ng = electric ? s->ngroups_e : s->ngroups_h;
+  if (ng == 0) return;
+  SrcArgs<T> a{};
+  a.fa = field_args<T>(s);
+  a.groups = s->d_groups + (electric ? 0 : s->ngroups_e);
+  a.ngroups = ng;
+  a.ops = static_cast<const DevSrcOp<R>*>(s->d_ops);
+  a.waves = s->d_waves;
+  a.aux = static_cast<void* const*>(s->d_aux_ptrs);
+  a.ctl = reinterpret_cast<Ctl*>(s->d_clock);
+  time_begin(s, 3, st);
+  launch_src<T>(a, st);
+  time_end(s, st);
+}
+
+template <class T>
+void enqueue_probe(gmes_sim* s, bool electric, hipStream_t st) {
+  if (s->probes.empty()) return;
+  ProbeArgs<T> a{};
+  a.fa = field_args<T>(s);
+  a.cells = s->d_probe_cells;
+  a.comps = s->d_probe_comps;
+  a.n = (int)s->probes.size();
+  a.electric = electric;
+  a.buf = s->d_probe_buf;
+  a.cap = s->probe_cap;
+  a.ctl = reinterpret_cast<Ctl*>(s->d_clock);
+  launch_probe<T>(a, st);
+}
+
+template <class T>
+void enqueue_step_t(gmes_sim* s, hipStream_t st) {
+  enqueue_ghost<T>(s, false, 1, st);
+  enqueue_main<T>(s, true, st);
+  enqueue_ade<T>(s, st);
+  enqueue_src<T>(s, true, st);
+  enqueue_probe<T>(s, true, st);
+  for (int i = 0; i < s->naux; ++i) {
+    gmes_sim* a = s->aux[i];
+    time_begin(s, 5, st);
+    switch (a->g.dtype) {
+      case GMES_F32: enqueue_step_t<float>(a, st); break;
+      case GMES_F64: enqueue_step_t<double>(a, st); break;
+      case GMES_C64: enqueue_step_t<cplx<float>>(a, st); break;
+      default: enqueue_step_t<cplx<double>>(a, st); break;
+    }
+    time_end(s, st);
+  }
+  enqueue_ghost<T>(s, true, 0, st);
+  enqueue_main<T>(s, false, st);
+  enqueue_src<T>(s, false, st);
+  enqueue_probe<T>(s, false, st);
+}
+
+void enqueue_step(gmes_sim* s, hipStream_t st) {
+  switch (s->g.dtype) {
+    case GMES_F32: enqueue_step_t<float>(s, st); break;
+    case GMES_F64: enqueue_step_t<double>(s, st); break;
+    case GMES_C64: enqueue_step_t<cplx<float>>(s, st); break;
+    default: enqueue_step_t<cplx<double>>(s, st); break;
+  }
+}
+
+void advance_host_clock(gmes_sim* s, int64_t n) {
+  s->step += n;
+  for (int i = 0; i < s->naux; ++i) advance_host_clock(s->aux[i], n);
+}
+
+// Checks that every step in [s->step, s->step + n) is covered by the
+// waveform window and the probe buffer (recursively for aux sims).
+int check_windows(gmes_sim* s, int64_t n) {
+  bool waves = false;
+  for (const auto& o : s->ops)
+    if (o.kind == GMES_SRC_SET || o.kind == GMES_SRC_SUB) waves = true;
+  if (waves && (s->step < s->wave_step0 || s->step + n > s->wave_step0 + s->wave_n))
+    return set_error(GMES_EINVAL, "waveforms cover steps [%lld, %lld), need [%lld, %lld)",
+                     (long long)s->wave_step0, (long long)(s->wave_step0 + s->wave_n),
+                     (long long)s->step, (long long)(s->step + n));
+  for (int i = 0; i < s->naux; ++i) {
+    int rc = check_windows(s->aux[i], n);
+    if (rc) return rc;
+  }
+  return GMES_OK;
+}
+
+int ensure_probe_capacity(gmes_sim* s, int64_t n) {
+  if (s->probes.empty()) return GMES_OK;
+  const int64_t need = (s->step - s->probe_base) + n;
+  if (need <= s->probe_cap) return GMES_OK;
+  int64_t cap = std::max<int64_t>(1024, s->probe_cap);
+  while (cap < need) cap *= 2;
+  double* nb = nullptr;
+  int rc = dev_alloc(s, (void**)&nb, (size_t)cap * s->probes.size() * 2 * sizeof(double));
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (s->d_probe_buf) {
+    const int64_t used = s->step - s->probe_base;
+    for (size_t p = 0; p < s->probes.size(); ++p)
+      HIP_TRY(hipMemcpy(nb + p * cap * 2, s->d_probe_buf + p * s->probe_cap * 2, used * 2 * sizeof(double),
+                        hipMemcpyDeviceToDevice));
+    dev_free(s, s->d_probe_buf, (size_t)s->probe_cap * s->probes.size() * 2 * sizeof(double));
+  }
+  s->d_probe_buf = nb;
+  s->probe_cap = cap;
+  s->graph_dirty = true;
+  return GMES_OK;
+}
+
+void destroy_graph(gmes_sim* s) {
+  if (s->graph_exec) {
+    (void)hipGraphExecDestroy(s->graph_exec);
+    s->graph_exec = nullptr;
+  }
+  s->graph_dirty = true;
+}
+
+int build_graph(gmes_sim* s) {
+  destroy_graph(s);
+  hipGraph_t graph;
+  HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  enqueue_step(s, s->stream);
+  HIP_TRY(hipStreamEndCapture(s->stream, &graph));
+  hipError_t e = hipGraphInstantiate(&s->graph_exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return set_error(GMES_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  s->graph_dirty = false;
+  return GMES_OK;
+}
+
+int collect_timing(gmes_sim* s) {
+  for (auto& p : s->timing.pending) {
+    float ms = 0.f;
+    HIP_TRY(hipEventSynchronize(p.second.second));
+    HIP_TRY(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+    s->timing.ms[p.first] += ms;
+    s->timing.launches[p.first] += 1;
+    (void)hipEventDestroy(p.second.first);
+    (void)hipEventDestroy(p.second.second);
+  }
+  s->timing.pending.clear();
+  return GMES_OK;
+}
+
+bool any_graph_blocker(gmes_sim* s) { return s->timing.on; }
+
+int do_step(gmes_sim* s, int64_t n) {
+  if (!s->committed) return set_error(GMES_EINVAL, "sim is not committed");
+  if (n < 0) return set_error(GMES_EINVAL, "negative step count");
+  int rc = check_windows(s, n);
+  if (rc) return rc;
+  if ((rc = ensure_probe_capacity(s, n))) return rc;
+  HIP_TRY(hipSetDevice(s->device));
+  if (s->use_graph && !any_graph_blocker(s)) {
+    if (s->graph_dirty || !s->graph_exec)
+      if ((rc = build_graph(s))) return rc;
+    for (int64_t t = 0; t < n; ++t) HIP_TRY(hipGraphLaunch(s->graph_exec, s->stream));
+  } else {
+    for (int64_t t = 0; t < n; ++t) enqueue_step(s, s->stream);
+    HIP_TRY(hipGetLastError());
+  }
+  advance_host_clock(s, n);
+  s->probe_n = s->step - s->probe_base;
+  if (s->timing.on && s->timing.pending.size() > 4096) return collect_timing(s);
+  return GMES_OK;
+}
+
+int ctl_write(gmes_sim* s) {
+  Ctl c;
+  HIP_TRY(hipMemcpyAsync(&c, s->d_clock, sizeof c, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  c.wave_step0 = s->wave_step0;
+  c.wave_n = s->wave_n;
+  c.probe_base = s->probe_base;
+  HIP_TRY(hipMemcpyAsync(s->d_clock, &c, sizeof c, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return GMES_OK;
+}
+
+}  // namespace
+
+// =============================================================== C-ABI
+extern "C" {
+
+const char* gmes_last_error(void) { return g_err; }
+int gmes_abi_version(void) { return GMES_ABI_VERSION; }
+
+int gmes_device_count(int32_t* count) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return GMES_OK;
+}
+
+int gmes_sim_create(const gmes_grid_desc* desc, gmes_sim** out) {
+  if (!desc || !out) return set_error(GMES_EINVAL, "null argument");
+  if (desc->dtype < GMES_F32 || desc->dtype > GMES_C128) return set_error(GMES_ETYPE, "bad dtype");
+  if (desc->nx < 1 || desc->ny < 1 || desc->nz < 1) return set_error(GMES_EINVAL, "grid sizes must be positive");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(GMES_EDEVICE, "no GPU visible");
+  if (desc->device < 0 || desc->device >= ndev) return set_error(GMES_EINVAL, "bad device ordinal");
+  HIP_TRY(hipSetDevice(desc->device));
+  gmes_sim* s = new gmes_sim();
+  s->g = *desc;
+  s->device = desc->device;
+  s->complex_ = desc->dtype == GMES_C64 || desc->dtype == GMES_C128;
+  s->fp64 = desc->dtype == GMES_F64 || desc->dtype == GMES_C128;
+  s->esize = (s->fp64 ? 8 : 4) * (s->complex_ ? 2 : 1);
+  const int64_t n[3] = {desc->nx, desc->ny, desc->nz};
+  for (int c = 0; c < 6; ++c) {
+    if (!active(s, c)) {
+      s->shape[c][0] = s->shape[c][1] = s->shape[c][2] = 1;
+    } else {
+      for (int a = 0; a < 3; ++a) s->shape[c][a] = n[a] + 1;
+      s->shape[c][c % 3] = n[c % 3];
+    }
+    s->ncell[c] = s->shape[c][0] * s->shape[c][1] * s->shape[c][2];
+    if (s->ncell[c] >= (int64_t(1) << 31)) {
+      delete s;
+      return set_error(GMES_EINVAL, "component %d has more than 2^31 cells", c);
+    }
+  }
+  int rc = GMES_OK;
+  for (int c = 0; c < 6 && !rc; ++c) {
+    rc = dev_alloc(s, &s->f[c], (size_t)(s->ncell[c] * s->esize));
+    if (!rc && hipMemset(s->f[c], 0, (size_t)(s->ncell[c] * s->esize)) != hipSuccess)
+      rc = set_error(GMES_EDEVICE, "hipMemset failed");
+  }
+  if (!rc) rc = dev_alloc(s, (void**)&s->d_clock, sizeof(Ctl));
+  if (!rc && hipMemset(s->d_clock, 0, sizeof(Ctl)) != hipSuccess) rc = set_error(GMES_EDEVICE, "hipMemset failed");
+  if (!rc) rc = dev_alloc(s, &s->d_aux_ptrs, 48 * sizeof(void*));
+  if (!rc && hipStreamCreate(&s->stream) != hipSuccess) rc = set_error(GMES_EDEVICE, "hipStreamCreate failed");
+  if (rc) {
+    gmes_sim_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return GMES_OK;
+}
+
+int gmes_sim_destroy(gmes_sim* s) {
+  if (!s) return GMES_OK;
+  (void)hipSetDevice(s->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  destroy_graph(s);
+  collect_timing(s);
+  for (int c = 0; c < 6; ++c) {
+    if (s->f[c]) (void)hipFree(s->f[c]);
+    if (s->ids[c]) (void)hipFree(s->ids[c]);
+    if (s->pml_state[c]) (void)hipFree(s->pml_state[c]);
+  }
+  free_ade(s);
+  void* bufs[] = {s->d_recs, s->d_coef, s->d_prof, s->d_prof_off, s->d_ops, s->d_groups, s->d_waves,
+                  s->d_aux_ptrs, s->d_probe_cells, s->d_probe_comps, s->d_probe_buf, s->d_clock};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return GMES_OK;
+}
+
+int gmes_sim_shape(const gmes_sim* s, int32_t c, int64_t shape[3]) {
+  if (c < 0 || c > 5) return set_error(GMES_EINVAL, "bad component");
+  for (int a = 0; a < 3; ++a) shape[a] = s->shape[c][a];
+  return GMES_OK;
+}
+
+int gmes_sim_footprint(const gmes_sim* s, int64_t* bytes) {
+  *bytes = s->bytes;
+  return GMES_OK;
+}
+
+int gmes_sim_set_materials(gmes_sim* s, const gmes_mat_rec* recs, int32_t nrec, const double* coef,
+                           int64_t ncoef) {
+  if (nrec < 1 || nrec > 65536) return set_error(GMES_EINVAL, "record count must be in [1, 65536]");
+  if (ncoef < 0) return set_error(GMES_EINVAL, "negative coefficient count");
+  s->recs.assign(recs, recs + nrec);
+  s->coef.assign(coef, coef + ncoef);
+  for (int r = 0; r < nrec; ++r) {
+    const gmes_mat_rec& m = recs[r];
+    if (m.cls < 0 || m.cls >= GMES_NCLS) return set_error(GMES_EINVAL, "record %d: bad class", r);
+    if (m.n1 < 0 || m.n2 < 0) return set_error(GMES_EINVAL, "record %d: bad pole count", r);
+    int64_t need = 0;
+    switch (m.cls) {
+      case GMES_CLS_DRUDE: need = 3 * (int64_t)m.n1 + 3; break;
+      case GMES_CLS_DCP_ADE: need = 3 * (int64_t)m.n1 + 5 * (int64_t)m.n2 + 4; break;
+      case GMES_CLS_DCP_PLRC: need = 3 * (int64_t)m.n1 + 6 * (int64_t)m.n2 + 3; break;
+      case GMES_CLS_DM2: need = 2 * (int64_t)m.n1 + 6; break;
+      default: break;
+    }
+    if (need && (m.coef_off < 0 || m.coef_off + need > ncoef))
+      return set_error(GMES_EINVAL, "record %d: coefficients out of the pool", r);
+    if (m.cls == GMES_CLS_DM2 && s->complex_) return set_error(GMES_EINVAL, "Dm2 class supports real fields only");
+    if (m.cls == GMES_CLS_DM2 && (m.n1 < 1 || m.n1 > kDm2Max))
+      return set_error(GMES_EINVAL, "record %d: DM2 supports 1..%d transitions", r, kDm2Max);
+    if ((m.cls == GMES_CLS_CPML || m.cls == GMES_CLS_UPML) && (m.prof < 0 || m.prof >= 64))
+      return set_error(GMES_EINVAL, "record %d: bad profile set", r);
+    if (m.cls != GMES_CLS_DUMMY && m.cls != GMES_CLS_CONST && !(m.eps_mu != 0.0))
+      return set_error(GMES_EINVAL, "record %d: zero permittivity/permeability", r);
+  }
+  s->committed = false;
+  return GMES_OK;
+}
+
+int gmes_sim_set_material_ids(gmes_sim* s, int32_t c, const uint16_t* ids, int64_t count) {
+  if (c < 0 || c > 5) return set_error(GMES_EINVAL, "bad component");
+  if (count != s->ncell[c]) return set_error(GMES_EINVAL, "id count %lld != cells %lld", (long long)count,
+                                             (long long)s->ncell[c]);
+  s->host_ids[c].assign(ids, ids + count);
+  if (!s->ids[c]) {
+    int rc = dev_alloc(s, (void**)&s->ids[c], (size_t)count * sizeof(uint16_t));
+    if (rc) return rc;
+  }
+  HIP_TRY(hipMemcpy(s->ids[c], ids, (size_t)count * sizeof(uint16_t), hipMemcpyHostToDevice));
+  s->committed = false;
+  return GMES_OK;
+}
+
+int gmes_sim_set_profile(gmes_sim* s, int32_t p, int32_t c, int32_t axis, const double* vals, int32_t n,
+                         int32_t width) {
+  if (p < 0 || p >= 64 || c < 0 || c > 5 || axis < 0 || axis > 2)
+    return set_error(GMES_EINVAL, "bad profile address");
+  if (n != s->shape[c][axis] || (width != 3 && width != 6)) return set_error(GMES_EINVAL, "bad profile size");
+  s->prof_host[p][c][axis].assign(vals, vals + (size_t)n * width);
+  s->prof_w[p][c][axis] = width;
+  s->nprof = std::max(s->nprof, p + 1);
+  s->committed = false;
+  return GMES_OK;
+}
+
+int gmes_sim_commit(gmes_sim* s) {
+  if (s->recs.empty()) return set_error(GMES_EINVAL, "materials not set");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  free_ade(s);
+  for (int c = 0; c < 6; ++c) {
+    for (int k = 0; k < GMES_NCLS; ++k) s->has_cls[c][k] = false;
+    if (!active(s, c)) continue;
+    const std::vector<uint16_t>& ids = s->host_ids[c];
+    if ((int64_t)ids.size() != s->ncell[c]) return set_error(GMES_EINVAL, "ids of component %d not set", c);
+    const int64_t sy = s->shape[c][1], sz = s->shape[c][2];
+    std::vector<std::vector<int32_t>> cells(GMES_NCLS);
+    std::vector<std::vector<uint16_t>> recs(GMES_NCLS);
+    std::vector<int> words(GMES_NCLS, 0);
+    for (int64_t x = 0; x < s->ncell[c]; ++x) {
+      const uint16_t id = ids[x];
+      if (id >= s->recs.size()) return set_error(GMES_ERANGE, "material id out of range");
+      const gmes_mat_rec& r = s->recs[id];
+      s->has_cls[c][r.cls] = true;
+      // Ghost planes must be Dummy: a kernel never reads past them
+      // (gmes/fdtd.py:365-366 and the analogous lines).
+      const int64_t ix[3] = {x / (sy * sz), (x / sz) % sy, x % sz};
+      bool ghost = false;
+      for (int a = 0; a < 3; ++a) {
+        if (a == c % 3) continue;
+        if (c < 3 ? ix[a] == s->shape[c][a] - 1 : ix[a] == 0) ghost = true;
+      }
+      if (ghost && r.cls != GMES_CLS_DUMMY)
+        return set_error(GMES_EINVAL, "component %d: ghost cell %lld is not Dummy", c, (long long)x);
+      if (r.cls >= GMES_CLS_DRUDE) {
+        if (c >= 3) return set_error(GMES_EINVAL, "dispersive class on a magnetic component");
+        cells[r.cls].push_back((int32_t)x);
+        recs[r.cls].push_back(id);
+        words[r.cls] = std::max(words[r.cls], state_words(r));
+      }
+      if (r.cls == GMES_CLS_CPML || r.cls == GMES_CLS_UPML) {
+        const int w = r.cls == GMES_CLS_CPML ? 3 : 6;
+        for (int a = 0; a < 3; ++a)
+          if ((int64_t)s->prof_host[r.prof][c][a].size() != s->shape[c][a] * w)
+            return set_error(GMES_EINVAL, "profile %d of component %d axis %d missing", r.prof, c, a);
+      }
+    }
+    // full-grid CPML (2 words) / UPML (1 word) state
+    const int pw = s->has_cls[c][GMES_CLS_CPML] ? 2 : (s->has_cls[c][GMES_CLS_UPML] ? 1 : 0);
+    if (s->pml_state[c]) { dev_free(s, s->pml_state[c], 0); s->pml_state[c] = nullptr; }
+    if (pw) {
+      const size_t b = (size_t)pw * s->ncell[c] * s->esize;
+      int rc = dev_alloc(s, &s->pml_state[c], b);
+      if (rc) return rc;
+      HIP_TRY(hipMemset(s->pml_state[c], 0, b));
+    }
+    for (int k = GMES_CLS_DRUDE; k < GMES_NCLS; ++k) {
+      AdeList& L = s->ade[c][k];
+      L.n = (int64_t)cells[k].size();
+      if (!L.n) continue;
+      L.nstate = words[k];
+      HIP_TRY(hipMalloc((void**)&L.cell, L.n * sizeof(int32_t)));
+      HIP_TRY(hipMalloc((void**)&L.rec, L.n * sizeof(uint16_t)));
+      const size_t sb = std::max<size_t>(1, (size_t)L.nstate * L.n * ade_word(s, k));
+      HIP_TRY(hipMalloc(&L.state, sb));
+      s->bytes += L.n * 6 + (int64_t)sb;
+      HIP_TRY(hipMemcpy(L.cell, cells[k].data(), L.n * sizeof(int32_t), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(L.rec, recs[k].data(), L.n * sizeof(uint16_t), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemset(L.state, 0, sb));
+    }
+  }
+  if (s->d_recs) { (void)hipFree(s->d_recs); s->d_recs = nullptr; }
+  if (s->d_coef) { (void)hipFree(s->d_coef); s->d_coef = nullptr; }
+  if (s->d_prof) { (void)hipFree(s->d_prof); s->d_prof = nullptr; }
+  if (s->d_prof_off) { (void)hipFree(s->d_prof_off); s->d_prof_off = nullptr; }
+  int rc = s->fp64 ? upload_tables<double>(s) : upload_tables<float>(s);
+  if (rc) return rc;
+  s->committed = true;
+  destroy_graph(s);
+  return GMES_OK;
+}
+
+int gmes_sim_set_sources(gmes_sim* s, const gmes_src_op* ops, int64_t nops, int32_t nwaves) {
+  if (nops < 0 || nwaves < 0) return set_error(GMES_EINVAL, "negative count");
+  if (nops > (1 << 30)) return set_error(GMES_EINVAL, "too many source operations");
+  for (int64_t i = 0; i < nops; ++i) {
+    const gmes_src_op& o = ops[i];
+    if (o.comp < 0 || o.comp > 5 || !active(s, o.comp)) return set_error(GMES_EINVAL, "source on an inactive component");
+    if (o.cell < 0 || o.cell >= s->ncell[o.comp]) return set_error(GMES_ERANGE, "field index is out of bounds");
+    if (o.kind == GMES_SRC_SET || o.kind == GMES_SRC_SUB) {
+      if (o.wave < 0 || o.wave >= nwaves) return set_error(GMES_EINVAL, "bad waveform row");
+    } else if (o.kind == GMES_SRC_AUX_ADD || o.kind == GMES_SRC_AUX_SUB) {
+      if (o.aux < 0 || o.aux >= s->naux) return set_error(GMES_EINVAL, "auxiliary grid %d not attached", o.aux);
+      if (o.aux_comp < 0 || o.aux_comp > 5) return set_error(GMES_EINVAL, "bad aux component");
+      const gmes_sim* a = s->aux[o.aux];
+      if (!((a->g.active >> o.aux_comp) & 1u)) return set_error(GMES_EINVAL, "inactive aux component");
+      if (o.i0 < 0 || o.i1 < 0 || o.i0 >= a->ncell[o.aux_comp] || o.i1 >= a->ncell[o.aux_comp])
+        return set_error(GMES_ERANGE, "aux sample index out of bounds");
+    } else {
+      return set_error(GMES_EINVAL, "bad source kind");
+    }
+  }
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  s->ops.assign(ops, ops + nops);
+  s->nwaves = nwaves;
+  int rc = s->fp64 ? upload_sources<double>(s) : upload_sources<float>(s);
+  destroy_graph(s);
+  return rc;
+}
+
+int gmes_sim_set_waveforms(gmes_sim* s, int64_t step0, int64_t nsteps, const double* values) {
+  if (nsteps < 0) return set_error(GMES_EINVAL, "negative step count");
+  HIP_TRY(hipSetDevice(s->device));
+  const int64_t need = std::max<int64_t>(1, (int64_t)s->nwaves * nsteps * 2);
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (need > s->wave_cap) {
+    if (s->d_waves) dev_free(s, s->d_waves, (size_t)s->wave_cap * sizeof(double));
+    s->d_waves = nullptr;
+    int rc = dev_alloc(s, (void**)&s->d_waves, (size_t)need * sizeof(double));
+    if (rc) return rc;
+    s->wave_cap = need;
+    destroy_graph(s);
+  }
+  if (s->nwaves && nsteps)
+    HIP_TRY(hipMemcpy(s->d_waves, values, (size_t)s->nwaves * nsteps * 2 * sizeof(double), hipMemcpyHostToDevice));
+  s->wave_step0 = step0;
+  s->wave_n = nsteps;
+  return ctl_write(s);
+}
+
+int gmes_sim_attach_aux(gmes_sim* s, gmes_sim* aux, int32_t* slot) {
+  if (!aux || aux == s) return set_error(GMES_EINVAL, "bad auxiliary sim");
+  if (s->naux >= 8) return set_error(GMES_EINVAL, "too many auxiliary grids");
+  if (aux->g.dtype != s->g.dtype) return set_error(GMES_ETYPE, "aux dtype mismatch");
+  if (aux->device != s->device) return set_error(GMES_EINVAL, "aux sim on another device");
+  if (aux->naux) return set_error(GMES_EINVAL, "an auxiliary grid cannot have its own");
+  if (aux->step != s->step) return set_error(GMES_EINVAL, "aux sim is at another step");
+  s->aux[s->naux] = aux;
+  void* ptrs[6];
+  for (int c = 0; c < 6; ++c) ptrs[c] = aux->f[c];
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipMemcpy(static_cast<void**>(s->d_aux_ptrs) + s->naux * 6, ptrs, sizeof ptrs, hipMemcpyHostToDevice));
+  *slot = s->naux++;
+  destroy_graph(s);
+  return GMES_OK;
+}
+
+int gmes_sim_add_probe(gmes_sim* s, int32_t c, int64_t cell, int32_t* id) {
+  if (c < 0 || c > 5 || !active(s, c) || cell < 0 || cell >= s->ncell[c])
+    return set_error(GMES_ERANGE, "probe index out of bounds");
+  if (s->step != s->probe_base) return set_error(GMES_EINVAL, "read pending probe values first");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  s->probes.push_back({c, cell});
+  std::vector<int64_t> cells;
+  std::vector<int32_t> comps;
+  for (auto& p : s->probes) { cells.push_back(p.cell); comps.push_back(p.comp); }
+  if (s->d_probe_cells) (void)hipFree(s->d_probe_cells);
+  if (s->d_probe_comps) (void)hipFree(s->d_probe_comps);
+  if (s->d_probe_buf) (void)hipFree(s->d_probe_buf);
+  s->d_probe_buf = nullptr;
+  s->probe_cap = 0;
+  HIP_TRY(hipMalloc((void**)&s->d_probe_cells, cells.size() * sizeof(int64_t)));
+  HIP_TRY(hipMalloc((void**)&s->d_probe_comps, comps.size() * sizeof(int32_t)));
+  HIP_TRY(hipMemcpy(s->d_probe_cells, cells.data(), cells.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(s->d_probe_comps, comps.data(), comps.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  *id = (int32_t)s->probes.size() - 1;
+  destroy_graph(s);
+  return GMES_OK;
+}
+
+static std::vector<int64_t>& probe_cursor(gmes_sim* s) {
+  static thread_local std::vector<std::pair<gmes_sim*, std::vector<int64_t>>> table;
+  for (auto& t : table)
+    if (t.first == s) return t.second;
+  table.push_back({s, {}});
+  return table.back().second;
+}
+
+int gmes_sim_read_probe(gmes_sim* s, int32_t id, double* out, int64_t max, int64_t* n) {
+  if (id < 0 || id >= (int32_t)s->probes.size()) return set_error(GMES_EINVAL, "bad probe id");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  std::vector<int64_t>& cur = probe_cursor(s);
+  cur.resize(s->probes.size(), 0);
+  const int64_t avail = s->probe_n - cur[id];
+  const int64_t m = std::min(avail, max);
+  if (m > 0)
+    HIP_TRY(hipMemcpy(out, s->d_probe_buf + (id * s->probe_cap + cur[id]) * 2, m * 2 * sizeof(double),
+                      hipMemcpyDeviceToHost));
+  cur[id] += m;
+  *n = m;
+  bool all = true;
+  for (size_t p = 0; p < cur.size(); ++p) all = all && cur[p] == s->probe_n;
+  if (all) {
+    s->probe_base = s->step;
+    s->probe_n = 0;
+    std::fill(cur.begin(), cur.end(), 0);
+    return ctl_write(s);
+  }
+  return GMES_OK;
+}
+
+int gmes_sim_write_field(gmes_sim* s, int32_t c, const void* host, int64_t bytes) {
+  if (c < 0 || c > 5) return set_error(GMES_EINVAL, "bad component");
+  if (bytes != s->ncell[c] * s->esize) return set_error(GMES_EINVAL, "size mismatch");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipMemcpyAsync(s->f[c], host, (size_t)bytes, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return GMES_OK;
+}
+
+int gmes_sim_read_field(gmes_sim* s, int32_t c, void* host, int64_t bytes) {
+  if (c < 0 || c > 5) return set_error(GMES_EINVAL, "bad component");
+  if (bytes != s->ncell[c] * s->esize) return set_error(GMES_EINVAL, "size mismatch");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipMemcpyAsync(host, s->f[c], (size_t)bytes, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return GMES_OK;
+}
+
+static void state_blocks(gmes_sim* s, std::vector<std::pair<void*, size_t>>& out) {
+  for (int c = 0; c < 6; ++c) {
+    if (s->pml_state[c]) {
+      const int pw = s->has_cls[c][GMES_CLS_CPML] ? 2 : 1;
+      out.push_back({s->pml_state[c], (size_t)pw * s->ncell[c] * s->esize});
+    }
+    for (int k = GMES_CLS_DRUDE; k < GMES_NCLS; ++k)
+      if (s->ade[c][k].n) out.push_back({s->ade[c][k].state, (size_t)s->ade[c][k].nstate * s->ade[c][k].n * ade_word(s, k)});
+  }
+}
+
+int gmes_sim_state_bytes(const gmes_sim* s, int64_t* bytes) {
+  std::vector<std::pair<void*, size_t>> b;
+  state_blocks(const_cast<gmes_sim*>(s), b);
+  int64_t t = 0;
+  for (auto& x : b) t += (int64_t)x.second;
+  *bytes = t;
+  return GMES_OK;
+}
+
+int gmes_sim_snapshot(gmes_sim* s, void* host, int64_t bytes) {
+  std::vector<std::pair<void*, size_t>> b;
+  state_blocks(s, b);
+  int64_t t = 0;
+  for (auto& x : b) t += (int64_t)x.second;
+  if (t != bytes) return set_error(GMES_EINVAL, "size mismatch");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  char* p = static_cast<char*>(host);
+  for (auto& x : b) { HIP_TRY(hipMemcpy(p, x.first, x.second, hipMemcpyDeviceToHost)); p += x.second; }
+  return GMES_OK;
+}
+
+int gmes_sim_restore(gmes_sim* s, const void* host, int64_t bytes) {
+  std::vector<std::pair<void*, size_t>> b;
+  state_blocks(s, b);
+  int64_t t = 0;
+  for (auto& x : b) t += (int64_t)x.second;
+  if (t != bytes) return set_error(GMES_EINVAL, "size mismatch");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  const char* p = static_cast<const char*>(host);
+  for (auto& x : b) { HIP_TRY(hipMemcpy(x.first, p, x.second, hipMemcpyHostToDevice)); p += x.second; }
+  return GMES_OK;
+}
+
+int gmes_sim_step(gmes_sim* s, int64_t nsteps) { return do_step(s, nsteps); }
+
+int gmes_sim_sync(gmes_sim* s) {
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  Ctl c;
+  HIP_TRY(hipMemcpy(&c, s->d_clock, sizeof c, hipMemcpyDeviceToHost));
+  if (c.err == 1) return set_error(GMES_ERUNTIME, "Dm2 corrector produced an invalid error");
+  if (c.err == 2) return set_error(GMES_ERUNTIME, "Dm2 corrector failed to converge");
+  for (int i = 0; i < s->naux; ++i) {
+    int rc = gmes_sim_sync(s->aux[i]);
+    if (rc) return rc;
+  }
+  if (s->timing.on) return collect_timing(s);
+  return GMES_OK;
+}
+
+int gmes_sim_get_step(const gmes_sim* s, int64_t* step) {
+  *step = s->step;
+  return GMES_OK;
+}
+
+int gmes_sim_set_graph(gmes_sim* s, int32_t enable) {
+  s->use_graph = enable != 0;
+  destroy_graph(s);
+  return GMES_OK;
+}
+
+int gmes_sim_set_tile(gmes_sim* s, int32_t bx, int32_t by) {
+  if (bx < 0 || by < 0 || bx * by > 1024 || (bx && (bx & (bx - 1))))
+    return set_error(GMES_EINVAL, "bad tile");
+  s->tile_bx = bx;
+  s->tile_by = by;
+  destroy_graph(s);
+  return GMES_OK;
+}
+
+int gmes_sim_profile(gmes_sim* s, int32_t enable) {
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  collect_timing(s);
+  s->timing.on = enable != 0;
+  for (int i = 0; i < 6; ++i) { s->timing.ms[i] = 0; s->timing.launches[i] = 0; }
+  return GMES_OK;
+}
+
+int gmes_sim_read_profile(gmes_sim* s, double ms[6], int64_t launches[6]) {
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  int rc = collect_timing(s);
+  if (rc) return rc;
+  for (int i = 0; i < 6; ++i) { ms[i] = s->timing.ms[i]; launches[i] = s->timing.launches[i]; }
+  return GMES_OK;
+}
+
+// ----------------------------------------------------------------- batches
+struct gmes_batch {
+  std::vector<gmes_sim*> sims;
+  bool graph = false;
+};
+
+int gmes_batch_create(gmes_sim* const* sims, int32_t n, gmes_batch** out) {
+  if (n < 1) return set_error(GMES_EINVAL, "empty batch");
+  for (int i = 0; i < n; ++i)
+    if (!sims[i] || sims[i]->device != sims[0]->device) return set_error(GMES_EINVAL, "batch members must share a device");
+  gmes_batch* b = new gmes_batch();
+  b->sims.assign(sims, sims + n);
+  *out = b;
+  return GMES_OK;
+}
+
+int gmes_batch_destroy(gmes_batch* b) {
+  delete b;
+  return GMES_OK;
+}
+
+int gmes_batch_set_graph(gmes_batch* b, int32_t enable) {
+  b->graph = enable != 0;
+  for (auto* s : b->sims) gmes_sim_set_graph(s, enable);
+  return GMES_OK;
+}
+
+// Every member runs on its own stream; the launches of one step are issued
+// round-robin over the members so that their kernels overlap on the GPU.
+int gmes_batch_step(gmes_batch* b, int64_t nsteps) {
+  for (auto* s : b->sims) {
+    int rc = check_windows(s, nsteps);
+    if (rc) return rc;
+    if ((rc = ensure_probe_capacity(s, nsteps))) return rc;
+    if (!s->committed) return set_error(GMES_EINVAL, "sim is not committed");
+  }
+  for (auto* s : b->sims)
+    if (s->use_graph && (s->graph_dirty || !s->graph_exec)) {
+      int rc = build_graph(s);
+      if (rc) return rc;
+    }
+  HIP_TRY(hipSetDevice(b->sims[0]->device));
+  for (int64_t t = 0; t < nsteps; ++t)
+    for (auto* s : b->sims) {
+      if (s->use_graph) HIP_TRY(hipGraphLaunch(s->graph_exec, s->stream));
+      else enqueue_step(s, s->stream);
+    }
+  HIP_TRY(hipGetLastError());
+  for (auto* s : b->sims) {
+    advance_host_clock(s, nsteps);
+    s->probe_n = s->step - s->probe_base;
+  }
+  return GMES_OK;
+}
+
+int gmes_batch_sync(gmes_batch* b) {
+  for (auto* s : b->sims) {
+    int rc = gmes_sim_sync(s);
+    if (rc) return rc;
+  }
+  return GMES_OK;
+}
+
+}  // extern "C"

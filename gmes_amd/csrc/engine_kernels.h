@@ -1,0 +1,104 @@
+// Kernel argument blocks and launchers shared by kernels.hip and engine.hip.
+#pragma once
+
+#include "engine_internal.h"
+
+namespace gmes_engine {
+
+constexpr int kDm2Max = 8;  // transitions per DM2 cell held in registers
+
+// Per-sim control block in device memory: everything a step reads that
+// changes between steps, so one captured step graph can be replayed.
+struct Ctl {
+  int64_t clock;       // steps started (thread 0 of the first kernel of a step adds 1)
+  int64_t wave_step0;  // absolute step of waveform row 0
+  int64_t wave_n;      // rows per waveform
+  int64_t probe_base;  // absolute step of probe slot 0
+  int32_t err;         // deferred device error (1 NaN, 2 no convergence)
+  int32_t pad_;
+};
+
+template <class T>
+struct FieldArgs {
+  T* f[6];
+  int s[6][3];
+  unsigned active;
+};
+
+template <class T>
+struct MainArgs {
+  using R = typename real_of<T>::type;
+  FieldArgs<T> fa;
+  const uint16_t* ids[6];
+  const DevRec<R>* recs;
+  const R* prof;
+  const int64_t* prof_off;
+  void* pml[6];
+  int ncell[6];
+  R d[3];
+};
+
+template <class T>
+struct AdeArgs {
+  using R = typename real_of<T>::type;
+  FieldArgs<T> fa;
+  const int32_t* cell;
+  const uint16_t* rec;
+  void* state;
+  int n;
+  const DevRec<R>* recs;
+  const R* coef;
+  R d[3];
+  R dt;
+  Ctl* ctl;
+};
+
+template <class T>
+struct SrcArgs {
+  using R = typename real_of<T>::type;
+  FieldArgs<T> fa;
+  const SrcGroup* groups;
+  int ngroups;
+  const DevSrcOp<R>* ops;
+  const double* waves;
+  void* const* aux;
+  Ctl* ctl;
+};
+
+template <class T>
+struct ProbeArgs {
+  FieldArgs<T> fa;
+  const int64_t* cells;
+  const int32_t* comps;
+  int n;
+  int electric;
+  double* buf;
+  int64_t cap;
+  Ctl* ctl;
+};
+
+template <class T>
+struct GhostArgs {
+  using R = typename real_of<T>::type;
+  FieldArgs<T> fa;
+  int ncomp;
+  int comp[3];
+  int ax[3][3];      // a1, a2, own axis
+  int64_t count[3];  // ghost points per component
+  cplx<R> ph[3][2];  // phase of the copy along a1, a2
+  int tick;
+  Ctl* ctl;
+};
+
+template <class T>
+void launch_main(bool electric, const MainArgs<T>& m, int ext[3], int bx, int by, hipStream_t st);
+template <class T>
+void launch_ade(int cls, int c, const AdeArgs<T>& m, hipStream_t st);
+template <class T>
+void launch_src(const SrcArgs<T>& m, hipStream_t st);
+template <class T>
+void launch_probe(const ProbeArgs<T>& m, hipStream_t st);
+template <class T>
+void launch_ghost(const GhostArgs<T>& m, int64_t total, hipStream_t st);
+
+}  // namespace gmes_engine

@@ -1,0 +1,539 @@
+// CDNA4 kernels of the FDTD engine.
+//
+// Data layout in HBM: each Yee component is its own C-order array on the
+// reference's staggered storage shape (gmes/geometry.py:276-358), complex
+// values interleaved (re, im) like numpy.  A component's cells carry a
+// uint16 material id; the id selects a record (class + coefficients) from
+// a small table.  CPML/UPML cells keep their auxiliary words in full-grid
+// arrays of the component; dispersive (ADE) cells live in compact
+// per-class lists with structure-of-arrays state.
+//
+// Kernels per leapfrog step (gmes/fdtd.py:1099-1122):
+//   k_ghost<H>  periodic/Bloch ghost planes of H     (gmes/fdtd.py:955-1097)
+//   k_main<E>   curl + Dielectric/CPML/UPML/Const    (src/pw_{dielectric,cpml,upml,const}.hh)
+//   k_ade_*     Drude/Lorentz, DCP-ADE, DCP-PLRC, DM2 (src/pw_{drude,lorentz,dcp,dm2}.hh)
+//   k_src<E>    point sources, TFSF face corrections (gmes/pw_source.py)
+//   k_probe<E>
+//   (auxiliary 1-D grids)
+//   k_ghost<E>, k_main<H>, k_src<H>, k_probe<H>
+// Every FP64 expression keeps the reference's operation order (division by
+// d1/d2, dt/eps first, left-to-right sums) and the build disables FMA
+// contraction, so the FP64 path is bit-identical to the reference's.
+#include "engine_kernels.h"
+
+namespace gmes_engine {
+
+// ---------------------------------------------------------------- arithmetic
+template <class R> __device__ __forceinline__ R vadd(R a, R b) { return a + b; }
+template <class R> __device__ __forceinline__ R vsub(R a, R b) { return a - b; }
+template <class R> __device__ __forceinline__ R vscl(R s, R a) { return s * a; }
+template <class R> __device__ __forceinline__ R vdiv(R a, R d) { return a / d; }
+template <class R> __device__ __forceinline__ cplx<R> vadd(cplx<R> a, cplx<R> b) { return {a.re + b.re, a.im + b.im}; }
+template <class R> __device__ __forceinline__ cplx<R> vsub(cplx<R> a, cplx<R> b) { return {a.re - b.re, a.im - b.im}; }
+template <class R> __device__ __forceinline__ cplx<R> vscl(R s, cplx<R> a) { return {s * a.re, s * a.im}; }
+template <class R> __device__ __forceinline__ cplx<R> vdiv(cplx<R> a, R d) { return {a.re / d, a.im / d}; }
+template <class R> __device__ __forceinline__ cplx<R> cmul(cplx<R> a, cplx<R> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <class T> __device__ __forceinline__ T vzero() { return T{}; }
+template <class R> __device__ __forceinline__ R re_of(R a) { return a; }
+template <class R> __device__ __forceinline__ R re_of(cplx<R> a) { return a.re; }
+template <class R> __device__ __forceinline__ R im_of(R) { return R(0); }
+template <class R> __device__ __forceinline__ R im_of(cplx<R> a) { return a.im; }
+template <class T> __device__ __forceinline__ T make_val(typename real_of<T>::type re, typename real_of<T>::type im);
+template <> __device__ __forceinline__ float make_val<float>(float re, float) { return re; }
+template <> __device__ __forceinline__ double make_val<double>(double re, double) { return re; }
+template <> __device__ __forceinline__ cplx<float> make_val<cplx<float>>(float re, float im) { return {re, im}; }
+template <> __device__ __forceinline__ cplx<double> make_val<cplx<double>>(double re, double im) { return {re, im}; }
+
+// component geometry: in1, in2, d1 axis, d2 axis, own axis (gmes/fdtd.py:673-809)
+__device__ __constant__ int c_ax1[6] = {1, 2, 0, 1, 2, 0};
+__device__ __constant__ int c_ax2[6] = {2, 0, 1, 2, 0, 1};
+
+template <class T>
+__device__ __forceinline__ T ld(const FieldArgs<T>& a, int c, int i, int j, int k) {
+  // an inactive component is (1,1,1) and reads as zero (src/cpp23_support.hh:52-74)
+  if (!(a.active & (1u << c))) return vzero<T>();
+  return a.f[c][(i * a.s[c][1] + j) * a.s[c][2] + k];
+}
+
+// Curl differences D1, D2 of component C at (i,j,k): src/pw_dielectric.hh
+// :118-119 (Ex) :157-158 (Ey) :197-198 (Ez) :292-293 (Hx) :331-332 (Hy) :370-371 (Hz)
+template <int C, class T>
+__device__ __forceinline__ void curl_terms(const FieldArgs<T>& a, int i, int j, int k, T& D1, T& D2) {
+  if constexpr (C == 0) {
+    D1 = vsub(ld(a, 5, i + 1, j + 1, k), ld(a, 5, i + 1, j, k));
+    D2 = vsub(ld(a, 4, i + 1, j, k + 1), ld(a, 4, i + 1, j, k));
+  } else if constexpr (C == 1) {
+    D1 = vsub(ld(a, 3, i, j + 1, k + 1), ld(a, 3, i, j + 1, k));
+    D2 = vsub(ld(a, 5, i + 1, j + 1, k), ld(a, 5, i, j + 1, k));
+  } else if constexpr (C == 2) {
+    D1 = vsub(ld(a, 4, i + 1, j, k + 1), ld(a, 4, i, j, k + 1));
+    D2 = vsub(ld(a, 3, i, j + 1, k + 1), ld(a, 3, i, j, k + 1));
+  } else if constexpr (C == 3) {
+    D1 = vsub(ld(a, 2, i, j, k - 1), ld(a, 2, i, j - 1, k - 1));
+    D2 = vsub(ld(a, 1, i, j - 1, k), ld(a, 1, i, j - 1, k - 1));
+  } else if constexpr (C == 4) {
+    D1 = vsub(ld(a, 0, i - 1, j, k), ld(a, 0, i - 1, j, k - 1));
+    D2 = vsub(ld(a, 2, i, j, k - 1), ld(a, 2, i - 1, j, k - 1));
+  } else {
+    D1 = vsub(ld(a, 1, i, j - 1, k), ld(a, 1, i - 1, j - 1, k));
+    D2 = vsub(ld(a, 0, i - 1, j, k), ld(a, 0, i - 1, j - 1, k));
+  }
+}
+
+template <int C> struct Ax;
+template <> struct Ax<0> { static constexpr int a1 = 1, a2 = 2, a0 = 0; };
+template <> struct Ax<1> { static constexpr int a1 = 2, a2 = 0, a0 = 1; };
+template <> struct Ax<2> { static constexpr int a1 = 0, a2 = 1, a0 = 2; };
+template <> struct Ax<3> { static constexpr int a1 = 1, a2 = 2, a0 = 0; };
+template <> struct Ax<4> { static constexpr int a1 = 2, a2 = 0, a0 = 1; };
+template <> struct Ax<5> { static constexpr int a1 = 0, a2 = 1, a0 = 2; };
+
+// ------------------------------------------------------------- main update
+// One thread per (i,j,k); every component of the field type whose storage
+// contains (i,j,k) is updated.  ADE classes are skipped here.
+template <int C, class T>
+__device__ __forceinline__ void main_cell(const MainArgs<T>& m, int i, int j, int k) {
+  using R = typename real_of<T>::type;
+  const FieldArgs<T>& a = m.fa;
+  const int flat = (i * a.s[C][1] + j) * a.s[C][2] + k;
+  const DevRec<R>& rec = m.recs[m.ids[C][flat]];
+  const int cls = rec.cls;
+  constexpr bool E = C < 3;
+  if (cls == GMES_CLS_DUMMY || cls >= GMES_CLS_DRUDE) return;
+  T* F = a.f[C] + flat;
+  if (cls == GMES_CLS_CONST) {  // src/pw_const.hh:96,198
+    *F = make_val<T>(rec.val_re, rec.val_im);
+    return;
+  }
+  T D1, D2;
+  curl_terms<C>(a, i, j, k, D1, D2);
+  const R d1 = m.d[Ax<C>::a1], d2 = m.d[Ax<C>::a2];
+  T f = *F;
+  if (cls == GMES_CLS_DIELECTRIC) {  // src/pw_dielectric.hh:118,292
+    if constexpr (E) f = vadd(f, vscl(rec.c0, vsub(vdiv(D1, d1), vdiv(D2, d2))));
+    else f = vadd(f, vscl(rec.c0, vsub(vdiv(D2, d2), vdiv(D1, d1))));
+  } else {
+    const int ix[3] = {i, j, k};
+    const int64_t* po = m.prof_off + (rec.prof * 6 + C) * 3;
+    T* st = static_cast<T*>(m.pml[C]);
+    if (cls == GMES_CLS_CPML) {  // src/pw_cpml.hh:127-132,339-344
+      const R* p1 = m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 3;
+      const R* p2 = m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 3;
+      T psi1 = st[flat], psi2 = st[m.ncell[C] + flat];
+      psi1 = vadd(vscl(p1[0], psi1), vdiv(vscl(p1[1], D1), d1));
+      psi2 = vadd(vscl(p2[0], psi2), vdiv(vscl(p2[1], D2), d2));
+      st[flat] = psi1;
+      st[m.ncell[C] + flat] = psi2;
+      T t = vsub(vadd(vsub(vdiv(vdiv(D1, d1), p1[2]), vdiv(vdiv(D2, d2), p2[2])), psi1), psi2);
+      if constexpr (E) f = vadd(f, vscl(rec.c0, t));
+      else f = vsub(f, vscl(rec.c0, t));
+    } else {  // GMES_CLS_UPML, src/pw_upml.hh:124-128,328-332
+      const R* p1 = m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 6;
+      const R* p2 = m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 6;
+      const R* p0 = m.prof + po[Ax<C>::a0] + ix[Ax<C>::a0] * 6;
+      T dstore = st[flat], dn;
+      T cu = vsub(vdiv(D1, d1), vdiv(D2, d2));
+      if constexpr (E) dn = vadd(vscl(p1[0], dstore), vscl(p1[1], cu));
+      else dn = vsub(vscl(p1[0], dstore), vscl(p1[1], cu));
+      st[flat] = dn;
+      f = vadd(vscl(p2[2], f), vdiv(vscl(p2[3], vsub(vscl(p0[4], dn), vscl(p0[5], dstore))), rec.eps_mu));
+    }
+  }
+  *F = f;
+}
+
+template <bool E, class T>
+__global__ void __launch_bounds__(256) k_main(MainArgs<T> m) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = blockIdx.z;
+  const FieldArgs<T>& a = m.fa;
+  constexpr int C0 = E ? 0 : 3;
+#define GMES_MAIN_COMP(CC)                                                              \
+  if ((a.active & (1u << (CC))) && i < a.s[CC][0] && j < a.s[CC][1] && k < a.s[CC][2]) \
+    main_cell<CC>(m, i, j, k);
+  GMES_MAIN_COMP(C0 + 0)
+  GMES_MAIN_COMP(C0 + 1)
+  GMES_MAIN_COMP(C0 + 2)
+#undef GMES_MAIN_COMP
+}
+
+// ----------------------------------------------------------------- ADE kernels
+template <class T>
+__device__ __forceinline__ void cell_ijk(const FieldArgs<T>& a, int c, int flat, int& i, int& j, int& k) {
+  const int sz = a.s[c][2], sy = a.s[c][1];
+  k = flat % sz;
+  const int t = flat / sz;
+  j = t % sy;
+  i = t / sy;
+}
+
+// Drude and Lorentz poles: src/pw_drude.hh:90-101,145-150; src/pw_lorentz.hh:96-107,150-156
+template <int C, class T>
+__global__ void __launch_bounds__(256) k_ade_drude(AdeArgs<T> m) {
+  using R = typename real_of<T>::type;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= m.n) return;
+  const int flat = m.cell[q];
+  int i, j, k;
+  cell_ijk(m.fa, C, flat, i, j, k);
+  const DevRec<R>& rec = m.recs[m.rec[q]];
+  const R* a = m.coef + rec.coef_off;
+  const int P = rec.n1;
+  const R* cc = a + 3 * P;
+  T* st = static_cast<T*>(m.state);
+  T* F = m.fa.f[C] + flat;
+  const T e_now = *F;
+  T sum = vzero<T>();
+  for (int p = 0; p < P; ++p) {
+    T q_old = st[p * m.n + q];
+    T q_now = st[(P + p) * m.n + q];
+    T q_new = vadd(vadd(vscl(a[3 * p], q_old), vscl(a[3 * p + 1], q_now)), vscl(a[3 * p + 2], e_now));
+    st[p * m.n + q] = q_now;
+    st[(P + p) * m.n + q] = q_new;
+  }
+  for (int p = 0; p < P; ++p) sum = vadd(sum, vsub(st[(P + p) * m.n + q], st[p * m.n + q]));
+  T D1, D2;
+  curl_terms<C>(m.fa, i, j, k, D1, D2);
+  const R d1 = m.d[Ax<C>::a1], d2 = m.d[Ax<C>::a2];
+  T cu = vsub(vdiv(D1, d1), vdiv(D2, d2));
+  *F = vadd(vadd(vscl(cc[0], cu), vscl(cc[1], sum)), vscl(cc[2], e_now));
+}
+
+// Drude-critical-points, ADE form: src/pw_dcp.hh:105-167,200-225
+template <int C, class T>
+__global__ void __launch_bounds__(256) k_ade_dcp(AdeArgs<T> m) {
+  using R = typename real_of<T>::type;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= m.n) return;
+  const int flat = m.cell[q];
+  int i, j, k;
+  cell_ijk(m.fa, C, flat, i, j, k);
+  const DevRec<R>& rec = m.recs[m.rec[q]];
+  const int D = rec.n1, NC = rec.n2;
+  const R* a = m.coef + rec.coef_off;
+  const R* b = a + 3 * D;
+  const R* cc = b + 5 * NC;
+  T* st = static_cast<T*>(m.state);
+  const int n = m.n;
+  // words: 0 e_old | 1..D q_old | D+1..2D q_now | 2D+1.. p_old | 2D+C+1.. p_now
+  T* F = m.fa.f[C] + flat;
+  const T e_now = *F;
+  const T e_old = st[q];
+  T ds = vzero<T>(), cs = vzero<T>();
+  for (int p = 0; p < D; ++p)
+    ds = vadd(ds, vsub(vscl(R(1) - a[3 * p + 1], st[(1 + D + p) * n + q]), vscl(a[3 * p], st[(1 + p) * n + q])));
+  for (int p = 0; p < NC; ++p)
+    cs = vadd(cs, vsub(vscl(R(1) - b[5 * p + 1], st[(1 + 2 * D + NC + p) * n + q]),
+                       vscl(b[5 * p], st[(1 + 2 * D + p) * n + q])));
+  T D1, D2;
+  curl_terms<C>(m.fa, i, j, k, D1, D2);
+  const R d1 = m.d[Ax<C>::a1], d2 = m.d[Ax<C>::a2];
+  T cu = vsub(vdiv(D1, d1), vdiv(D2, d2));
+  T e_new = vadd(vadd(vadd(vscl(cc[0], cu), vscl(cc[1], vadd(ds, cs))), vscl(cc[2], e_old)), vscl(cc[3], e_now));
+  T mix = vadd(vadd(e_old, vscl(R(2), e_now)), e_new);
+  for (int p = 0; p < D; ++p) {
+    T qo = st[(1 + p) * n + q], qn = st[(1 + D + p) * n + q];
+    T qq = vadd(vadd(vscl(a[3 * p], qo), vscl(a[3 * p + 1], qn)), vscl(a[3 * p + 2], mix));
+    st[(1 + p) * n + q] = qn;
+    st[(1 + D + p) * n + q] = qq;
+  }
+  for (int p = 0; p < NC; ++p) {
+    T po = st[(1 + 2 * D + p) * n + q], pn = st[(1 + 2 * D + NC + p) * n + q];
+    T pp = vadd(vadd(vadd(vadd(vscl(b[5 * p], po), vscl(b[5 * p + 1], pn)), vscl(b[5 * p + 2], e_old)),
+                     vscl(b[5 * p + 3], e_now)),
+                vscl(b[5 * p + 4], e_new));
+    st[(1 + 2 * D + p) * n + q] = pn;
+    st[(1 + 2 * D + NC + p) * n + q] = pp;
+  }
+  st[q] = e_now;
+  *F = e_new;
+}
+
+// Drude-critical-points, PLRC/RC form: src/pw_dcp.hh:467-518,564-586.
+// State words are complex for any field type: psi_dp (re part = psi_dp_re,
+// im part = psi_dp_im), psi_cp_re, psi_cp_im.
+template <int C, class T>
+__global__ void __launch_bounds__(256) k_ade_plrc(AdeArgs<T> m) {
+  using R = typename real_of<T>::type;
+  using Z = cplx<R>;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= m.n) return;
+  const int flat = m.cell[q];
+  int i, j, k;
+  cell_ijk(m.fa, C, flat, i, j, k);
+  const DevRec<R>& rec = m.recs[m.rec[q]];
+  const int D = rec.n1, NC = rec.n2;
+  const R* a = m.coef + rec.coef_off;
+  const R* b = a + 3 * D;
+  const R* cc = b + 6 * NC;
+  Z* st = static_cast<Z*>(m.state);
+  const int n = m.n;
+  T* F = m.fa.f[C] + flat;
+  const T fv = *F;
+  const Z e_now = {re_of(fv), im_of(fv)};
+  R sdr = R(0), sdi = R(0);
+  for (int p = 0; p < D; ++p) sdr = sdr + st[p * n + q].re;
+  for (int p = 0; p < D; ++p) sdi = sdi + st[p * n + q].im;
+  Z scr = {R(0), R(0)}, sci = {R(0), R(0)};
+  for (int p = 0; p < NC; ++p) scr = vadd(scr, st[(D + p) * n + q]);
+  for (int p = 0; p < NC; ++p) sci = vadd(sci, st[(D + NC + p) * n + q]);
+  const Z psi = {sdr + scr.re, sdi + sci.re};  // .real() of the imaginary-part sum
+  T D1, D2;
+  curl_terms<C>(m.fa, i, j, k, D1, D2);
+  const R d1 = m.d[Ax<C>::a1], d2 = m.d[Ax<C>::a2];
+  const T cu = vsub(vdiv(D1, d1), vdiv(D2, d2));
+  Z e_new;
+  if constexpr (is_cplx<T>::value) {
+    e_new = vadd(vadd(vscl(cc[0], Z{cu.re, cu.im}), vscl(cc[1], e_now)), vscl(cc[2], psi));
+  } else {
+    e_new.re = (cc[0] * cu + cc[1] * e_now.re) + cc[2] * psi.re;
+    e_new.im = cc[1] * e_now.im + cc[2] * psi.im;
+  }
+  for (int p = 0; p < D; ++p) {
+    Z v = st[p * n + q];
+    v.re = (a[3 * p] * e_new.re + a[3 * p + 1] * e_now.re) + a[3 * p + 2] * v.re;
+    v.im = (a[3 * p] * e_new.im + a[3 * p + 1] * e_now.im) + a[3 * p + 2] * v.im;
+    st[p * n + q] = v;
+  }
+  for (int p = 0; p < NC; ++p) {
+    const Z b0 = {b[6 * p], b[6 * p + 1]}, b1 = {b[6 * p + 2], b[6 * p + 3]}, b2 = {b[6 * p + 4], b[6 * p + 5]};
+    st[(D + p) * n + q] = vadd(vadd(vscl(e_new.re, b0), vscl(e_now.re, b1)), cmul(b2, st[(D + p) * n + q]));
+    st[(D + NC + p) * n + q] = vadd(vadd(vscl(e_new.im, b0), vscl(e_now.im, b1)), cmul(b2, st[(D + NC + p) * n + q]));
+  }
+  *F = make_val<T>(e_new.re, e_new.im);
+}
+
+// Two-level density matrix, predictor-corrector: src/pw_dm2.hh:345-407.
+// Real fields only; uses only the D2 curl term and no eps (reference quirk).
+template <class R>
+__device__ double dm2_l2(R e, const R* u, int n) {
+  R accu = e * e;
+  for (int i = 0; i < n; ++i) {
+    R ip = R(0);
+    for (int jj = 0; jj < 3; ++jj) ip = ip + u[3 * i + jj] * u[3 * i + jj];
+    accu += ip;
+  }
+  return sqrt((double)accu);
+}
+
+template <int C, class T>
+__global__ void __launch_bounds__(256) k_ade_dm2(AdeArgs<T> m) {
+  using R = typename real_of<T>::type;
+  if constexpr (is_cplx<T>::value) {
+    return;
+  } else {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m.n) return;
+    const int flat = m.cell[q];
+    int i, j, k;
+    cell_ijk(m.fa, C, flat, i, j, k);
+    const DevRec<R>& rec = m.recs[m.rec[q]];
+    const int N = rec.n1;
+    const R* cf = m.coef + rec.coef_off;
+    const R* omega = cf;
+    const R* n_atom = cf + N;
+    const R rho30 = cf[2 * N], gamma = cf[2 * N + 1], t1 = cf[2 * N + 2], t2 = cf[2 * N + 3];
+    const R hbar = cf[2 * N + 4];
+    const double rtol = (double)cf[2 * N + 5];
+    const int64_t step = m.ctl->clock - 1;
+    const R n_half = (R)((double)step + 0.5);
+    const R t = (n_half + R(0.5)) * m.dt;
+    R av[kDm2Max], bv[kDm2Max], u[3 * kDm2Max], u_new[3 * kDm2Max], u_tmp[3 * kDm2Max], dif[3 * kDm2Max];
+    for (int p = 0; p < N; ++p) {
+      av[p] = n_atom[p] * gamma / t2 * exp(-t / t2);
+      bv[p] = n_atom[p] * gamma * omega[p] * exp(-t / t2);
+    }
+    const R c_plus = R(2) / hbar * gamma * exp(-t * (R(1) / t1 - R(1) / t2));
+    const R c_minus = R(2) / hbar * gamma * exp(-t * (R(1) / t2 - R(1) / t1));
+    const R dd = R(2) / hbar * gamma * rho30 * exp(t / t2);
+    R* st = static_cast<R*>(m.state);
+    for (int p = 0; p < 3 * N; ++p) { u[p] = st[p * m.n + q]; u_new[p] = u[p]; }
+    T* F = m.fa.f[C] + flat;
+    R e_new = *F;
+    const R e_old = *F;
+    T D1, D2;
+    curl_terms<C>(m.fa, i, j, k, D1, D2);
+    const R h_d = D2 / m.d[Ax<C>::a2];
+    const R dt = m.dt;
+    double error;
+    int iteration = 0, fail = 0;
+    do {
+      const R e_tmp = e_new;
+      for (int p = 0; p < 3 * N; ++p) u_tmp[p] = u_new[p];
+      e_new = e_old - dt * h_d;
+      for (int p = 0; p < N; ++p) {
+        e_new -= R(.5) * dt * av[p] * (u_new[3 * p] + u[3 * p]);
+        e_new += R(.5) * dt * bv[p] * (u_new[3 * p + 1] + u[3 * p + 1]);
+      }
+      for (int p = 0; p < N; ++p) {
+        u_new[3 * p] = u[3 * p] + R(.5) * dt * omega[p] * (u_new[3 * p + 1] + u[3 * p + 1]);
+        u_new[3 * p + 1] = u[3 * p + 1] - R(.5) * dt * omega[p] * (u_new[3 * p] + u[3 * p]) +
+                           R(.25) * dt * c_plus * (u_new[3 * p + 2] + u[3 * p + 2]) * (e_new + e_old) +
+                           R(.5) * dt * dd * (e_new + e_old);
+        u_new[3 * p + 2] = u[3 * p + 2] - R(.25) * dt * c_minus * (u_new[3 * p + 1] + u[3 * p + 1]) * (e_new + e_old);
+      }
+      for (int p = 0; p < 3 * N; ++p) dif[p] = u_new[p] - u_tmp[p];
+      const double dn = dm2_l2(e_new - e_tmp, dif, N);
+      const double rn = dm2_l2(e_tmp, u_tmp, N);
+      error = rn == 0 ? (dn == 0 ? 0.0 : INFINITY) : dn / rn;
+      if (isnan(error)) { fail = 1; break; }
+      if (++iteration >= 100 && error > rtol) { fail = 2; break; }
+    } while (error > rtol);
+    if (fail) {
+      m.ctl->err = fail;  // plain vector store; any failing cell reports
+      return;
+    }
+    *F = e_new;
+    for (int p = 0; p < 3 * N; ++p) st[p * m.n + q] = u_new[p];
+  }
+}
+
+// ---------------------------------------------------------------- sources
+// One thread per (component, cell) group; its ops run in table order
+// (gmes/pw_source.py:74-87,106-119 point sources; :216-486 TFSF faces).
+template <class T>
+__global__ void __launch_bounds__(256) k_src(SrcArgs<T> m) {
+  using R = typename real_of<T>::type;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= m.ngroups) return;
+  const SrcGroup grp = m.groups[g];
+  T* F = m.fa.f[grp.comp] + grp.cell;
+  T v = *F;
+  const int64_t row = (m.ctl->clock - 1) - m.ctl->wave_step0;
+  for (int o = grp.begin; o < grp.end; ++o) {
+    const DevSrcOp<R> op = m.ops[o];
+    if (op.kind == GMES_SRC_SET || op.kind == GMES_SRC_SUB) {
+      const double* w = m.waves + (op.wave * m.ctl->wave_n + row) * 2;
+      const T wv = make_val<T>((R)w[0], (R)w[1]);
+      v = op.kind == GMES_SRC_SET ? wv : vsub(v, wv);
+    } else {
+      const T* aux = static_cast<const T*>(m.aux[op.aux * 6 + op.aux_comp]);
+      const T inc = vadd(vscl(op.r0, aux[op.i0]), vscl(op.r1, aux[op.i1]));
+      v = op.kind == GMES_SRC_AUX_ADD ? vadd(v, vscl(op.k, inc)) : vsub(v, vscl(op.k, inc));
+    }
+  }
+  *F = v;
+}
+
+template <class T>
+__global__ void k_probe(ProbeArgs<T> m) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= m.n) return;
+  const int c = m.comps[p];
+  if ((c < 3) != m.electric) return;
+  const T v = m.fa.f[c][m.cells[p]];
+  const int64_t pos = (m.ctl->clock - 1) - m.ctl->probe_base;
+  double* out = m.buf + (p * m.cap + pos) * 2;
+  out[0] = (double)re_of(v);
+  out[1] = (double)im_of(v);
+}
+
+// ------------------------------------------------------------- ghost planes
+// Periodic / Bloch ghost copy of every active component of one field type
+// (gmes/fdtd.py:811-1097).  The first copy along axis a1 and the second
+// along a2 are folded into one pass: every ghost point reads only interior
+// points, and the corner line gets ph2 * (ph1 * x), as the sequential
+// copies produce.  Thread 0 also advances the step clock (first kernel of
+// a step).
+template <class T>
+__global__ void __launch_bounds__(256) k_ghost(GhostArgs<T> m) {
+  using R = typename real_of<T>::type;
+  if (m.tick && blockIdx.x == 0 && threadIdx.x == 0) m.ctl->clock = m.ctl->clock + 1;
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int t = 0;
+  while (t < m.ncomp && e >= m.count[t]) { e -= m.count[t]; ++t; }
+  if (t >= m.ncomp) return;
+  const int c = m.comp[t];
+  const int a1 = m.ax[t][0], a2 = m.ax[t][1], a0 = m.ax[t][2];
+  const int* s = m.fa.s[c];
+  const bool electric = c < 3;
+  const int dst1 = electric ? s[a1] - 1 : 0, src1 = electric ? 0 : s[a1] - 1;
+  const int dst2 = electric ? s[a2] - 1 : 0, src2 = electric ? 0 : s[a2] - 1;
+  int ix[3];
+  T v;
+  const int64_t plane1 = (int64_t)s[a0] * s[a2];
+  T* f = m.fa.f[c];
+  if (e < plane1) {
+    ix[a0] = (int)(e / s[a2]);
+    ix[a2] = (int)(e % s[a2]);
+    ix[a1] = dst1;
+    int sx[3] = {ix[0], ix[1], ix[2]};
+    sx[a1] = src1;
+    if (ix[a2] == dst2) sx[a2] = src2;
+    v = f[(sx[0] * s[1] + sx[1]) * s[2] + sx[2]];
+    if constexpr (is_cplx<T>::value) {
+      v = cmul(m.ph[t][0], v);
+      if (ix[a2] == dst2) v = cmul(m.ph[t][1], v);
+    }
+  } else {
+    e -= plane1;
+    const int w1 = s[a1] - 1;
+    ix[a0] = (int)(e / w1);
+    const int w = (int)(e % w1);
+    ix[a1] = electric ? w : w + 1;
+    ix[a2] = dst2;
+    int sx[3] = {ix[0], ix[1], ix[2]};
+    sx[a2] = src2;
+    v = f[(sx[0] * s[1] + sx[1]) * s[2] + sx[2]];
+    if constexpr (is_cplx<T>::value) v = cmul(m.ph[t][1], v);
+  }
+  f[(ix[0] * s[1] + ix[1]) * s[2] + ix[2]] = v;
+}
+
+// --------------------------------------------------------------- launchers
+template <class T>
+void launch_main(bool electric, const MainArgs<T>& m, int ext[3], int bx, int by, hipStream_t st) {
+  dim3 block(bx, by, 1);
+  dim3 grid((ext[2] + bx - 1) / bx, (ext[1] + by - 1) / by, ext[0]);
+  if (electric) hipLaunchKernelGGL((k_main<true, T>), grid, block, 0, st, m);
+  else hipLaunchKernelGGL((k_main<false, T>), grid, block, 0, st, m);
+}
+
+template <class T>
+void launch_ade(int cls, int c, const AdeArgs<T>& m, hipStream_t st) {
+  dim3 block(256), grid((unsigned)((m.n + 255) / 256));
+#define GMES_ADE_SWITCH(KERNEL)                                               \
+  switch (c) {                                                               \
+    case 0: hipLaunchKernelGGL((KERNEL<0, T>), grid, block, 0, st, m); break; \
+    case 1: hipLaunchKernelGGL((KERNEL<1, T>), grid, block, 0, st, m); break; \
+    default: hipLaunchKernelGGL((KERNEL<2, T>), grid, block, 0, st, m); break; \
+  }
+  if (cls == GMES_CLS_DRUDE) { GMES_ADE_SWITCH(k_ade_drude) }
+  else if (cls == GMES_CLS_DCP_ADE) { GMES_ADE_SWITCH(k_ade_dcp) }
+  else if (cls == GMES_CLS_DCP_PLRC) { GMES_ADE_SWITCH(k_ade_plrc) }
+  else if (cls == GMES_CLS_DM2) { GMES_ADE_SWITCH(k_ade_dm2) }
+#undef GMES_ADE_SWITCH
+}
+
+template <class T>
+void launch_src(const SrcArgs<T>& m, hipStream_t st) {
+  hipLaunchKernelGGL((k_src<T>), dim3((m.ngroups + 255) / 256), dim3(256), 0, st, m);
+}
+
+template <class T>
+void launch_probe(const ProbeArgs<T>& m, hipStream_t st) {
+  hipLaunchKernelGGL((k_probe<T>), dim3((m.n + 63) / 64), dim3(64), 0, st, m);
+}
+
+template <class T>
+void launch_ghost(const GhostArgs<T>& m, int64_t total, hipStream_t st) {
+  const int64_t blocks = total > 0 ? (total + 255) / 256 : 1;
+  hipLaunchKernelGGL((k_ghost<T>), dim3((unsigned)blocks), dim3(256), 0, st, m);
+}
+
+#define GMES_INSTANTIATE(T)                                                                     \
+  template void launch_main<T>(bool, const MainArgs<T>&, int[3], int, int, hipStream_t);       \
+  template void launch_ade<T>(int, int, const AdeArgs<T>&, hipStream_t);                       \
+  template void launch_src<T>(const SrcArgs<T>&, hipStream_t);                                  \
+  template void launch_probe<T>(const ProbeArgs<T>&, hipStream_t);                              \
+  template void launch_ghost<T>(const GhostArgs<T>&, int64_t, hipStream_t);
+
+GMES_INSTANTIATE(float)
+GMES_INSTANTIATE(double)
+GMES_INSTANTIATE(cplx<float>)
+GMES_INSTANTIATE(cplx<double>)
+
+}  // namespace gmes_engine

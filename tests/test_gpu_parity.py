@@ -1,0 +1,179 @@
+"""Parity of the HIP engine (through the C-ABI) with the CPU oracle.
+
+FP64 runs must match the oracle to a relative L2 of 1e-12 over all six
+fields (the kernels keep the reference's operation order, so the match is
+normally exact).  FP32 runs are checked against the engine's own FP64 run
+of the same case: relative L2 <= 1e-5 unless a looser bound is stated
+with the case.
+"""
+
+import numpy as np
+import pytest
+
+import gmes_amd as g
+from cases import (air2d, bloch2d, bloch3d, const_case, currents, dcp_ade, dcp_plrc, dcp_rc,
+                   dispersive2d, dm2_1d, drude, fields, kat_tmz, lorentz, mode_case, rel_l2,
+                   spheres3d)
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+
+def run(builder, lib, steps, **kw):
+    sim = builder(lib, **kw)
+    for _ in range(steps):
+        sim.step()
+    out = fields(sim)
+    sim.close()
+    return out
+
+
+def test_kat_tmz_regression(engine_lib):
+    """The reference's TMz known answer (tests/test_fdtd.py:55-79)."""
+    sim = kat_tmz(engine_lib)
+    for _ in range(5):
+        sim.step()
+    assert sim.time_step.n == 5.0
+    assert abs(sim.time_step.t - 0.7000357133746822) < 1e-15
+    assert sim.ez.shape == (11, 11, 1)
+    assert sim.ez[5, 5, 0] == -0.9996801161298625
+    assert abs(np.sum(np.abs(sim.ez) ** 2) - 1.5780099423691636) < 1e-14
+
+
+def test_kat_tmz_matches_oracle_bitwise(engine_lib, oracle_lib):
+    a = run(kat_tmz, engine_lib, 5)
+    b = run(kat_tmz, oracle_lib, 5)
+    for c in a:
+        np.testing.assert_array_equal(a[c], b[c])
+
+
+def test_air2d_full_run(engine_lib, oracle_lib):
+    """Plain method, FP64: examples/air2d.py to t = 10 (286 steps)."""
+    sim = air2d(engine_lib)
+    sim.step_until_t(10)
+    assert sim.time_step.n == 286.0
+    gpu = fields(sim)
+    # examples/VERIFICATION.md:7 records the peak |Ez| of this run
+    assert abs(np.max(np.abs(gpu[g.Ez])) - 0.999382775451675) < 1e-12
+    ref = air2d(oracle_lib)
+    ref.step_until_t(10)
+    assert rel_l2(gpu, fields(ref)) <= FP64_TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_spheres3d(engine_lib, oracle_lib, precision):
+    steps = 60
+    gpu = run(spheres3d, engine_lib, steps, precision=precision)
+    if precision == "fp64":
+        assert rel_l2(gpu, run(spheres3d, oracle_lib, steps)) <= FP64_TOL
+    else:
+        assert rel_l2(gpu, run(spheres3d, engine_lib, steps)) <= 1e-5
+
+
+MODES = [g.TExFDTD, g.TEyFDTD, g.TEzFDTD, g.TMxFDTD, g.TMyFDTD, g.TMzFDTD, g.TEMxFDTD,
+         g.TEMyFDTD, g.TEMzFDTD]
+
+
+@pytest.mark.parametrize("mode", MODES, ids=lambda m: m.__name__)
+@pytest.mark.parametrize("pml", [g.Cpml, g.Upml], ids=["cpml", "upml"])
+def test_modes(engine_lib, oracle_lib, mode, pml):
+    gpu = run(lambda lib: mode_case(mode, lib, pml=pml), engine_lib, 40)
+    ref = run(lambda lib: mode_case(mode, lib, pml=pml), oracle_lib, 40)
+    assert rel_l2(gpu, ref) <= FP64_TOL
+
+
+@pytest.mark.parametrize("mat", [drude, lorentz, dcp_ade, dcp_plrc, dcp_rc],
+                         ids=["drude", "lorentz", "dcp_ade", "dcp_plrc", "dcp_rc"])
+@pytest.mark.parametrize("mode", [g.TMzFDTD, g.TEzFDTD], ids=["tmz", "tez"])
+def test_dispersive(engine_lib, oracle_lib, mat, mode):
+    steps = 50
+    gpu = run(lambda lib, **kw: dispersive2d(mat(), lib, mode=mode, **kw), engine_lib, steps)
+    ref = run(lambda lib, **kw: dispersive2d(mat(), lib, mode=mode, **kw), oracle_lib, steps)
+    assert rel_l2(gpu, ref) <= FP64_TOL
+    g32 = run(lambda lib, **kw: dispersive2d(mat(), lib, mode=mode, **kw), engine_lib, steps,
+              precision="fp32")
+    assert rel_l2(g32, gpu) <= 1e-5
+
+
+@pytest.mark.parametrize("mat", [drude, dcp_plrc], ids=["drude", "dcp_plrc"])
+def test_dispersive_bloch(engine_lib, oracle_lib, mat):
+    gpu = run(lambda lib: dispersive2d(mat(), lib, bloch=(0.2, 0.1, 0)), engine_lib, 40)
+    ref = run(lambda lib: dispersive2d(mat(), lib, bloch=(0.2, 0.1, 0)), oracle_lib, 40)
+    assert rel_l2(gpu, ref) <= FP64_TOL
+
+
+def test_const_and_upml3d(engine_lib, oracle_lib):
+    gpu = run(const_case, engine_lib, 30)
+    ref = run(const_case, oracle_lib, 30)
+    assert rel_l2(gpu, ref) <= FP64_TOL
+
+
+def test_dm2(engine_lib, oracle_lib):
+    gpu = run(dm2_1d, engine_lib, 80)
+    ref = run(dm2_1d, oracle_lib, 80)
+    assert rel_l2(gpu, ref) <= 1e-10  # device exp() may differ from libm by an ulp
+
+
+def test_dm2_failure_is_reported(engine_lib):
+    """Dm2(rtol=-1) never converges (tests/test_pw_dm2.py:95-101)."""
+    sim = dm2_1d(engine_lib, rtol=-1)
+    with pytest.raises(RuntimeError, match="failed to converge"):
+        for _ in range(40):
+            sim.step()
+        sim.sync()
+
+
+def test_currents_and_shared_cell(engine_lib, oracle_lib):
+    gpu = run(currents, engine_lib, 30)
+    ref = run(currents, oracle_lib, 30)
+    assert rel_l2(gpu, ref) <= FP64_TOL
+
+
+@pytest.mark.parametrize("builder", [bloch2d, bloch3d], ids=["2d", "3d"])
+def test_bloch(engine_lib, oracle_lib, builder):
+    gpu = run(builder, engine_lib, 40)
+    ref = run(builder, oracle_lib, 40)
+    assert rel_l2(gpu, ref) <= FP64_TOL
+    g32 = run(builder, engine_lib, 40, precision="fp32")
+    assert rel_l2(g32, gpu) <= 1e-5
+
+
+def test_graph_replay_matches_direct(engine_lib):
+    a = spheres3d(engine_lib, n=16, nsph=3, pml=3)
+    b = spheres3d(engine_lib, n=16, nsph=3, pml=3)
+    engine_lib.set_graph(b._sim, 1)
+    for k in range(3):
+        a._advance(7)
+        b._advance(7)
+        fa, fb = fields(a), fields(b)
+        for c in fa:
+            np.testing.assert_array_equal(fa[c], fb[c])
+
+
+def test_step_batches_match_single_steps(engine_lib):
+    a = kat_tmz(engine_lib)
+    b = kat_tmz(engine_lib)
+    for _ in range(9):
+        a.step()
+    b._advance(9)
+    for c in g.constant.COMPONENTS:
+        np.testing.assert_array_equal(a.field[c], b.field[c])
+
+
+def test_host_write_before_step(engine_lib, oracle_lib):
+    """sim.ez[idx] = 1 before stepping reaches the device
+    (tests/test_fdtd.py:28-33 of the reference)."""
+    out = []
+    for lib in (engine_lib, oracle_lib):
+        sim = g.TMzFDTD(g.Cartesian(size=(2, 2, 0), resolution=3),
+                        [g.DefaultMedium(material=g.Dielectric())], verbose=False, _engine=lib)
+        sim.init()
+        idx = sim.space.space_to_ez_index(0, 0, 0)
+        sim.ez[idx] = 1
+        sim.step_while_zero(g.Ez, (0, 0, 0))
+        assert sim.time_step.n == 1
+        for _ in range(3):
+            sim.step()
+        out.append(fields(sim))
+    assert rel_l2(out[0], out[1]) <= FP64_TOL
