@@ -1,0 +1,195 @@
+#!/usr/bin/env python
+"""Benchmark of the FDTD field-update hot path on one MI355X.
+
+Workload (headline): 3-D full-vector FDTD on 256^3 cells -- background
+eps = 1, 40 seed-placed dielectric spheres (radius 4-20 cells, eps in
+[1.5, 12]), a 12-cell CPML on all faces, a Gaussian-pulse Ex point source
+at the centre, Courant 0.5 -- run in FP32 (the value) and in FP64 (the
+reference's precision).  A "step" is one full leapfrog step (E and H half
+steps with ghosts, CPML, source) over the whole grid; fields stay resident
+in HBM.  Metric: Mcell-updates/s = cells x steps / s.
+
+Printed as one JSON line on stdout; diagnostics go to stderr.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(sim, electric):
+    """Algorithmic HBM bytes of one main-kernel launch: every field word of
+    the updated type read and written once, every word of the other type
+    read once, and the two CPML auxiliary words of each CPML cell read and
+    written once.  Material ids and coefficient tables are not counted."""
+    from gmes_amd import constant as const
+    from gmes_amd._abi import CLS_CPML, NUMPY_DTYPE
+    s = NUMPY_DTYPE[sim.dtype_code]().itemsize
+    own = const.E_COMPONENTS if electric else const.H_COMPONENTS
+    other = const.H_COMPONENTS if electric else const.E_COMPONENTS
+    n_own = sum(sim._arrays[c].size for c in own if c in sim._active)
+    n_other = sum(sim._arrays[c].size for c in other if c in sim._active)
+    n_pml = sum(sim._tiles.class_counts[c].get(CLS_CPML, 0) for c in own if c in sim._active)
+    return s * (2 * n_own + n_other) + 4 * s * n_pml
+
+
+def time_steps(sim, lib, steps):
+    lib.sync(sim._sim)
+    t0 = time.perf_counter()
+    sim._advance(steps)
+    lib.sync(sim._sim)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(n, steps, warm):
+    """The reference's own updaters (oracle/_ref/libgmes_ref.so, compiled
+    from the reference's src/pw_*.hh) in the reference's step order, FP64,
+    OpenMP over the host cores."""
+    from gmes_amd import workloads
+    from gmes_amd._abi import Library
+    path = os.path.join(ROOT, "oracle", "_ref", "libgmes_ref.so")
+    kind = "reference"
+    if not os.path.exists(path):
+        path = os.path.join(ROOT, "oracle", "libgmes_oracle.so")
+        kind = "port"
+    lib = Library(path)
+    sim = workloads.spheres3d(n, precision="fp64", _engine=lib)
+    sim._advance(warm)
+    t0 = time.perf_counter()
+    sim._advance(steps)
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    return sim, dict(value=round(n ** 3 * steps / dt / 1e6, 2), unit="Mcell-updates/s",
+                     cores=threads if kind == "reference" else 1, kind=kind,
+                     sample="%d^3 spheres+CPML workload, FP64, %d steps after %d warm-up steps "
+                            "(%.1f s)" % (n, steps, warm, dt))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=256, help="cells per axis")
+    ap.add_argument("--fp64-steps", type=int, default=500)
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs one simulation per GPU on one-GPU boxes; "
+              "--gpus %d is not supported (use --gpus 1)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from gmes_amd import _abi, workloads
+    lib = _abi.engine()
+    n = args.n
+    cells = n ** 3
+
+    # ---- FP32 headline ---------------------------------------------------
+    t = time.perf_counter()
+    sim = workloads.spheres3d(n, precision="fp32")
+    log("fp32 setup %.1f s, footprint %.2f GB" % (time.perf_counter() - t,
+                                                  lib.footprint(sim._sim) / 1e9))
+    lib.set_graph(sim._sim, 0 if args.no_graph else 1)
+    sim._advance(args.warmup)
+    secs = time_steps(sim, lib, args.steps)
+    value = cells * args.steps / secs / 1e6
+    log("fp32: %d steps in %.3f s -> %.1f Mcell/s" % (args.steps, secs, value))
+
+    # ---- roofline of the dominant kernel (HIP events on the sim's stream)
+    prof_steps = min(args.steps, 200)
+    lib.profile(sim._sim, 1)
+    sim._advance(prof_steps)
+    lib.sync(sim._sim)
+    ms, launches = lib.read_profile(sim._sim)
+    lib.profile(sim._sim, 0)
+    fam = 0 if ms[0] >= ms[1] else 1
+    avg_ms = ms[fam] / max(1, launches[fam])
+    nbytes = kernel_bytes(sim, electric=(fam == 0))
+    achieved = nbytes / (avg_ms * 1e-3) / 1e9
+    other = 1 - fam
+    other_avg = ms[other] / max(1, launches[other])
+    other_gbps = kernel_bytes(sim, electric=(other == 0)) / (other_avg * 1e-3) / 1e9
+    log("kernel ms/launch: E %.4f H %.4f ghost %.4f src %.4f" % (
+        ms[0] / max(1, launches[0]), ms[1] / max(1, launches[1]),
+        ms[4] / max(1, launches[4]), ms[3] / max(1, launches[3])))
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBPS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBPS, 4), traffic=None,
+                    kernel="k_main<%s,float>" % ("E" if fam == 0 else "H"),
+                    bytes_per_launch=int(nbytes), avg_launch_ms=round(avg_ms, 5),
+                    other_kernel_gbps=round(other_gbps, 1))
+    sim.close()
+
+    # ---- FP64 (the reference's precision) --------------------------------
+    fp64 = None
+    if args.fp64_steps > 0:
+        sim64 = workloads.spheres3d(n, precision="fp64")
+        lib.set_graph(sim64._sim, 0 if args.no_graph else 1)
+        sim64._advance(min(args.warmup, 10))
+        s64 = time_steps(sim64, lib, args.fp64_steps)
+        fp64 = dict(value=round(cells * args.fp64_steps / s64 / 1e6, 1), unit="Mcell-updates/s",
+                    steps=args.fp64_steps, ms_per_step=round(s64 / args.fp64_steps * 1e3, 4))
+        log("fp64: %d steps in %.3f s -> %.1f Mcell/s" % (args.fp64_steps, s64, fp64["value"]))
+        sim64.close()
+
+    # ---- CPU baseline ----------------------------------------------------
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        warm = 2
+        ref_sim, cpu = cpu_baseline(n, args.cpu_steps, warm)
+        log("cpu baseline:", cpu)
+        # the same steps on the GPU in FP64 must reproduce the reference
+        import numpy as np
+        from gmes_amd import constant as const
+        g64 = workloads.spheres3d(n, precision="fp64")
+        g64._advance(warm + args.cpu_steps)
+        num = den = 0.0
+        for c in const.COMPONENTS:
+            a = np.asarray(g64.field[c], np.float64)
+            b = np.asarray(ref_sim.field[c], np.float64)
+            num += float(np.sum((a - b) ** 2))
+            den += float(np.sum(b ** 2))
+        parity = dict(steps=warm + args.cpu_steps, rel_l2_fp64_gpu_vs_reference=(num / den) ** 0.5 if den else 0.0)
+        log("parity:", parity)
+        g64.close()
+
+    out = {
+        "metric": "Mcell-updates/s and % HBM peak (FP32 batched vs FP64 plain); p50/p99 solve latency",
+        "value": round(value, 1),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(secs / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "3-D FDTD %d^3, 40 dielectric spheres, 12-cell CPML, Bandpass Ex "
+                               "source, Courant 0.5, FP32" % n, "cells": cells,
+                   "graph": not args.no_graph},
+        "hbm_pct": round(100 * achieved / HBM_PEAK_GBPS, 1),
+        "fp64": fp64,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "parity": parity,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -140,14 +140,15 @@ def build_tiles(space, geom_list, e_comps, h_comps, cmplx):
         shape = space.storage_shape(comp, tuple(e_comps) + tuple(h_comps))
         owner, under, coords = owner_map(space, comp, shape, geom_list)
         compound = np.array([isinstance(g.material, Compound) for g in geom_list])
-        key = owner.astype(np.int64) * (len(geom_list) + 1)
+        nkey = len(geom_list) + 1
+        key = owner * nkey
         comp_owner = compound[owner]
-        key[comp_owner] += under[comp_owner].astype(np.int64) + 1
-        uniq, inv = np.unique(key, return_inverse=True)
-        lut = np.empty(len(uniq), np.uint16)
+        key[comp_owner] += under[comp_owner] + 1
+        present = np.flatnonzero(np.bincount(key.ravel(), minlength=nkey * nkey))
+        lut = np.zeros(nkey * nkey, np.uint16)
         mats = {}
-        for n, kv in enumerate(uniq):
-            o, u = divmod(int(kv), len(geom_list) + 1)
+        for kv in present:
+            o, u = divmod(int(kv), nkey)
             mat = geom_list[o].material
             underneath = geom_list[u - 1].material if u > 0 else None
             tile = mat._tile(electric, underneath, cmplx)
@@ -160,9 +161,9 @@ def build_tiles(space, geom_list, e_comps, h_comps, cmplx):
                     if kk not in ts.profiles:
                         ts.profiles[kk] = pml.profile(coords[a], a)
             rid = ts.record_id(tile, pml_index)
-            lut[n] = rid
+            lut[kv] = rid
             mats[rid] = mat
-        ids = lut[inv.reshape(shape)]
+        ids = lut[key]
         for sl in ghost_mask_slices(comp, shape):
             ids[sl] = dummy_rid
         ts.ids[comp] = np.ascontiguousarray(ids)
