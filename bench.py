@@ -44,6 +44,20 @@ def kernel_bytes(sim, electric):
     return s * (2 * n_own + n_other) + 4 * s * n_pml
 
 
+def measured_traffic(n, fam):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-kernel scaled), for the
+    256^3 FP32 workload only; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "k_main3d_spheres256_fp32.md")
+    if n != 256 or not os.path.exists(path):
+        return None
+    key = "traffic_bytes_per_launch_%s:" % fam
+    for line in open(path):
+        if line.startswith(key):
+            return int(line.split(":")[1])
+    return None
+
+
 def time_steps(sim, lib, steps):
     lib.sync(sim._sim)
     t0 = time.perf_counter()
@@ -126,7 +140,8 @@ def main():
         ms[0] / max(1, launches[0]), ms[1] / max(1, launches[1]),
         ms[4] / max(1, launches[4]), ms[3] / max(1, launches[3])))
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBPS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBPS, 4), traffic=None,
+                    frac=round(achieved / HBM_PEAK_GBPS, 4),
+                    traffic=measured_traffic(n, "E" if fam == 0 else "H"),
                     kernel="k_main<%s,float>" % ("E" if fam == 0 else "H"),
                     bytes_per_launch=int(nbytes), avg_launch_ms=round(avg_ms, 5),
                     other_kernel_gbps=round(other_gbps, 1))

@@ -45,6 +45,16 @@ const int kGhostOrder[6][2] = {{1, 2}, {2, 0}, {0, 1}, {1, 2}, {2, 0}, {0, 1}};
 
 bool active(const gmes_sim* s, int c) { return (s->g.active >> c) & 1u; }
 
+// device element of logical (i,j,k) / of a logical C-order flat index
+int64_t dev_index(const gmes_sim* s, int c, int64_t i, int64_t j, int64_t k) {
+  return kPad + (i * s->shape[c][1] + j) * s->pitch[c] + k - s->koff[c];
+}
+int64_t dev_flat(const gmes_sim* s, int c, int64_t flat) {
+  const int64_t sz = s->shape[c][2], sy = s->shape[c][1];
+  return dev_index(s, c, flat / (sy * sz), (flat / sz) % sy, flat % sz);
+}
+
+
 int dev_alloc(gmes_sim* s, void** p, size_t bytes) {
   if (bytes == 0) bytes = 16;
   HIP_TRY(hipMalloc(p, bytes));
@@ -101,6 +111,7 @@ int upload_tables(gmes_sim* s) {
     dr[i].coef_off = r.coef_off;
     dr[i].c0 = (R)(s->g.dt / r.eps_mu);  // dt / eps_inf in FP64, rounded once
     dr[i].eps_mu = (R)r.eps_mu;
+    dr[i].inv_eps = (R)(1.0 / r.eps_mu);
     dr[i].val_re = (R)r.value[0];
     dr[i].val_im = (R)r.value[1];
   }
@@ -113,7 +124,12 @@ int upload_tables(gmes_sim* s) {
     for (int c = 0; c < 6; ++c)
       for (int a = 0; a < 3; ++a) {
         off[(p * 6 + c) * 3 + a] = (int64_t)pr.size();
-        for (double v : s->prof_host[p][c][a]) pr.push_back((R)v);
+        // FP32 tables hold 1/kappa in the CPML kappa column (the FP32 kernels
+        // multiply); FP64 tables hold the reference's values unchanged.
+        const std::vector<double>& src = s->prof_host[p][c][a];
+        const bool recip = !s->fp64 && s->prof_w[p][c][a] == 3;
+        for (size_t q = 0; q < src.size(); ++q)
+          pr.push_back((R)(recip && q % 3 == 2 ? 1.0 / src[q] : src[q]));
       }
   if (pr.empty()) pr.push_back(R(0));
   if (cf.empty()) cf.push_back(R(0));
@@ -149,9 +165,14 @@ int upload_sources(gmes_sim* s) {
     DevSrcOp<R>& d = dops[n];
     d.kind = o.kind; d.wave = o.wave; d.aux = o.aux; d.aux_comp = o.aux_comp;
     d.i0 = o.i0; d.i1 = o.i1; d.k = (R)o.k; d.r0 = (R)o.r0; d.r1 = (R)o.r1;
-    if (groups.empty() || groups.back().comp != o.comp || groups.back().cell != o.cell) {
+    if (o.kind == GMES_SRC_AUX_ADD || o.kind == GMES_SRC_AUX_SUB) {
+      d.i0 = dev_flat(s->aux[o.aux], o.aux_comp, o.i0);
+      d.i1 = dev_flat(s->aux[o.aux], o.aux_comp, o.i1);
+    }
+    const int64_t dcell = dev_flat(s, o.comp, o.cell);
+    if (groups.empty() || groups.back().comp != o.comp || groups.back().cell != dcell) {
       SrcGroup g;
-      g.comp = o.comp; g.cell = o.cell; g.begin = (int32_t)n; g.end = (int32_t)n + 1; g.pad_ = 0;
+      g.comp = o.comp; g.cell = dcell; g.begin = (int32_t)n; g.end = (int32_t)n + 1; g.pad_ = 0;
       groups.push_back(g);
       if (o.comp < 3) s->ngroups_e++; else s->ngroups_h++;
     } else {
@@ -177,6 +198,8 @@ FieldArgs<T> field_args(const gmes_sim* s) {
   for (int c = 0; c < 6; ++c) {
     a.f[c] = static_cast<T*>(s->f[c]);
     for (int d = 0; d < 3; ++d) a.s[c][d] = (int)s->shape[c][d];
+    a.p[c] = s->pitch[c];
+    a.koff[c] = s->koff[c];
   }
   a.active = s->g.active;
   return a;
@@ -243,13 +266,30 @@ void enqueue_main(gmes_sim* s, bool electric, hipStream_t st) {
   for (int c = 0; c < 6; ++c) {
     m.ids[c] = s->ids[c];
     m.pml[c] = s->pml_state[c];
-    m.ncell[c] = (int)s->ncell[c];
+    m.dsize[c] = (int)s->dsize[c];
   }
   m.recs = static_cast<const DevRec<R>*>(s->d_recs);
   m.prof = static_cast<const R*>(s->d_prof);
   m.prof_off = s->d_prof_off;
   m.d[0] = (R)s->g.dx; m.d[1] = (R)s->g.dy; m.d[2] = (R)s->g.dz;
+  m.inv_d[0] = (R)(1.0 / s->g.dx); m.inv_d[1] = (R)(1.0 / s->g.dy); m.inv_d[2] = (R)(1.0 / s->g.dz);
+  m.nrec = (int)s->recs.size();
+  m.n[0] = s->g.nx; m.n[1] = s->g.ny; m.n[2] = s->g.nz;
   int bx = s->tile_bx, by = s->tile_by;
+  if (s->g.active == 0x3Fu) {
+    // 3-D: (j,k) tiles marching along i; enough blocks to fill 256 CUs.
+    // The vector kernel needs every row padded and nz a multiple of 4.
+    if (bx <= 0 || by <= 0) {
+      bx = std::min(128, next_pow2(s->g.nz));
+      by = std::max(1, 256 / bx);
+    }
+    int ic = s->tile_ic;
+    if (ic <= 0) ic = 2;
+    time_begin(s, electric ? 0 : 1, st);
+    launch_main3d<T>(electric, m, bx, by, ic, st);
+    time_end(s, st);
+    return;
+  }
   if (bx <= 0 || by <= 0) {
     bx = std::min(64, next_pow2(ext[2]));
     by = std::max(1, std::min(256 / bx, next_pow2(ext[1])));
@@ -499,15 +539,20 @@ int gmes_sim_create(const gmes_grid_desc* desc, gmes_sim** out) {
       s->shape[c][c % 3] = n[c % 3];
     }
     s->ncell[c] = s->shape[c][0] * s->shape[c][1] * s->shape[c][2];
-    if (s->ncell[c] >= (int64_t(1) << 31)) {
+    const int64_t s2 = s->shape[c][2];
+    const bool padded = s2 >= 32;
+    s->pitch[c] = (int32_t)(padded ? (s2 + 31) / 32 * 32 : s2);
+    s->koff[c] = padded && (c == GMES_HX || c == GMES_HY) ? 1 : 0;
+    s->dsize[c] = 2 * kPad + s->shape[c][0] * s->shape[c][1] * s->pitch[c];
+    if (s->dsize[c] >= (int64_t(1) << 31)) {
       delete s;
       return set_error(GMES_EINVAL, "component %d has more than 2^31 cells", c);
     }
   }
   int rc = GMES_OK;
   for (int c = 0; c < 6 && !rc; ++c) {
-    rc = dev_alloc(s, &s->f[c], (size_t)(s->ncell[c] * s->esize));
-    if (!rc && hipMemset(s->f[c], 0, (size_t)(s->ncell[c] * s->esize)) != hipSuccess)
+    rc = dev_alloc(s, &s->f[c], (size_t)(s->dsize[c] * s->esize));
+    if (!rc && hipMemset(s->f[c], 0, (size_t)(s->dsize[c] * s->esize)) != hipSuccess)
       rc = set_error(GMES_EDEVICE, "hipMemset failed");
   }
   if (!rc) rc = dev_alloc(s, (void**)&s->d_clock, sizeof(Ctl));
@@ -592,10 +637,15 @@ int gmes_sim_set_material_ids(gmes_sim* s, int32_t c, const uint16_t* ids, int64
                                              (long long)s->ncell[c]);
   s->host_ids[c].assign(ids, ids + count);
   if (!s->ids[c]) {
-    int rc = dev_alloc(s, (void**)&s->ids[c], (size_t)count * sizeof(uint16_t));
+    int rc = dev_alloc(s, (void**)&s->ids[c], (size_t)s->dsize[c] * sizeof(uint16_t));
     if (rc) return rc;
+    HIP_TRY(hipMemset(s->ids[c], 0, (size_t)s->dsize[c] * sizeof(uint16_t)));
   }
-  HIP_TRY(hipMemcpy(s->ids[c], ids, (size_t)count * sizeof(uint16_t), hipMemcpyHostToDevice));
+  const size_t row = (size_t)s->shape[c][2] * sizeof(uint16_t);
+  if (s->pitch[c] == s->shape[c][2] && s->koff[c] == 0)
+    HIP_TRY(hipMemcpy(s->ids[c] + kPad, ids, (size_t)count * sizeof(uint16_t), hipMemcpyHostToDevice));
+  else HIP_TRY(hipMemcpy2D(s->ids[c] + kPad - s->koff[c], (size_t)s->pitch[c] * sizeof(uint16_t), ids, row, row,
+                      (size_t)(s->shape[c][0] * s->shape[c][1]), hipMemcpyHostToDevice));
   s->committed = false;
   return GMES_OK;
 }
@@ -658,7 +708,7 @@ int gmes_sim_commit(gmes_sim* s) {
     const int pw = s->has_cls[c][GMES_CLS_CPML] ? 2 : (s->has_cls[c][GMES_CLS_UPML] ? 1 : 0);
     if (s->pml_state[c]) { dev_free(s, s->pml_state[c], 0); s->pml_state[c] = nullptr; }
     if (pw) {
-      const size_t b = (size_t)pw * s->ncell[c] * s->esize;
+      const size_t b = (size_t)pw * s->dsize[c] * s->esize;
       int rc = dev_alloc(s, &s->pml_state[c], b);
       if (rc) return rc;
       HIP_TRY(hipMemset(s->pml_state[c], 0, b));
@@ -764,7 +814,7 @@ int gmes_sim_add_probe(gmes_sim* s, int32_t c, int64_t cell, int32_t* id) {
   s->probes.push_back({c, cell});
   std::vector<int64_t> cells;
   std::vector<int32_t> comps;
-  for (auto& p : s->probes) { cells.push_back(p.cell); comps.push_back(p.comp); }
+  for (auto& p : s->probes) { cells.push_back(dev_flat(s, p.comp, p.cell)); comps.push_back(p.comp); }
   if (s->d_probe_cells) (void)hipFree(s->d_probe_cells);
   if (s->d_probe_comps) (void)hipFree(s->d_probe_comps);
   if (s->d_probe_buf) (void)hipFree(s->d_probe_buf);
@@ -815,7 +865,16 @@ int gmes_sim_write_field(gmes_sim* s, int32_t c, const void* host, int64_t bytes
   if (c < 0 || c > 5) return set_error(GMES_EINVAL, "bad component");
   if (bytes != s->ncell[c] * s->esize) return set_error(GMES_EINVAL, "size mismatch");
   HIP_TRY(hipSetDevice(s->device));
-  HIP_TRY(hipMemcpyAsync(s->f[c], host, (size_t)bytes, hipMemcpyHostToDevice, s->stream));
+  const size_t row = (size_t)s->shape[c][2] * s->esize;
+  if (s->pitch[c] == s->shape[c][2] && s->koff[c] == 0) {
+    HIP_TRY(hipMemcpyAsync(static_cast<char*>(s->f[c]) + (size_t)kPad * s->esize, host, (size_t)bytes,
+                           hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return GMES_OK;
+  }
+  HIP_TRY(hipMemcpy2DAsync(static_cast<char*>(s->f[c]) + (size_t)(kPad - s->koff[c]) * s->esize,
+                           (size_t)s->pitch[c] * s->esize, host, row, row,
+                           (size_t)(s->shape[c][0] * s->shape[c][1]), hipMemcpyHostToDevice, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   return GMES_OK;
 }
@@ -824,7 +883,16 @@ int gmes_sim_read_field(gmes_sim* s, int32_t c, void* host, int64_t bytes) {
   if (c < 0 || c > 5) return set_error(GMES_EINVAL, "bad component");
   if (bytes != s->ncell[c] * s->esize) return set_error(GMES_EINVAL, "size mismatch");
   HIP_TRY(hipSetDevice(s->device));
-  HIP_TRY(hipMemcpyAsync(host, s->f[c], (size_t)bytes, hipMemcpyDeviceToHost, s->stream));
+  const size_t row = (size_t)s->shape[c][2] * s->esize;
+  if (s->pitch[c] == s->shape[c][2] && s->koff[c] == 0) {
+    HIP_TRY(hipMemcpyAsync(host, static_cast<const char*>(s->f[c]) + (size_t)kPad * s->esize, (size_t)bytes,
+                           hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return GMES_OK;
+  }
+  HIP_TRY(hipMemcpy2DAsync(host, row, static_cast<const char*>(s->f[c]) + (size_t)(kPad - s->koff[c]) * s->esize,
+                           (size_t)s->pitch[c] * s->esize, row, (size_t)(s->shape[c][0] * s->shape[c][1]),
+                           hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   return GMES_OK;
 }
@@ -833,7 +901,7 @@ static void state_blocks(gmes_sim* s, std::vector<std::pair<void*, size_t>>& out
   for (int c = 0; c < 6; ++c) {
     if (s->pml_state[c]) {
       const int pw = s->has_cls[c][GMES_CLS_CPML] ? 2 : 1;
-      out.push_back({s->pml_state[c], (size_t)pw * s->ncell[c] * s->esize});
+      out.push_back({s->pml_state[c], (size_t)pw * s->dsize[c] * s->esize});
     }
     for (int k = GMES_CLS_DRUDE; k < GMES_NCLS; ++k)
       if (s->ade[c][k].n) out.push_back({s->ade[c][k].state, (size_t)s->ade[c][k].nstate * s->ade[c][k].n * ade_word(s, k)});
@@ -904,10 +972,12 @@ int gmes_sim_set_graph(gmes_sim* s, int32_t enable) {
 }
 
 int gmes_sim_set_tile(gmes_sim* s, int32_t bx, int32_t by) {
-  if (bx < 0 || by < 0 || bx * by > 1024 || (bx && (bx & (bx - 1))))
-    return set_error(GMES_EINVAL, "bad tile");
+  if (bx < 0 || by < 0 || bx * by > 256 || (bx && (bx & (bx - 1))))
+    return set_error(GMES_EINVAL, "bad tile: bx must be a power of two and bx*by <= 256");
   s->tile_bx = bx;
   s->tile_by = by;
+  const char* ic = getenv("GMES_ICHUNK");
+  s->tile_ic = ic ? atoi(ic) : 0;
   destroy_graph(s);
   return GMES_OK;
 }

@@ -30,6 +30,7 @@ struct DevRec {
   int64_t coef_off;  // into the device coefficient pool (R)
   R c0;              // dt / eps_mu
   R eps_mu;
+  R inv_eps;         // 1 / eps_mu
   R val_re, val_im;  // CONST value
 };
 
@@ -76,6 +77,9 @@ struct gmes_sim {
   bool fp64 = true;
   int64_t shape[6][3];
   int64_t ncell[6];
+  int32_t pitch[6];          // device row pitch (elements)
+  int32_t koff[6];           // device k offset (see FieldArgs)
+  int64_t dsize[6];          // device elements per component
   void* f[6] = {};           // device fields
   uint16_t* ids[6] = {};     // device ids
   std::vector<uint16_t> host_ids[6];
@@ -124,7 +128,7 @@ struct gmes_sim {
   hipGraphExec_t graph_exec = nullptr;
   bool graph_dirty = true;
   // launch shape
-  int tile_bx = 0, tile_by = 0;
+  int tile_bx = 0, tile_by = 0, tile_ic = 0;
   gmes_engine::Timing timing;
   int64_t bytes = 0;
 };

@@ -5,7 +5,8 @@
 
 namespace gmes_engine {
 
-constexpr int kDm2Max = 8;  // transitions per DM2 cell held in registers
+constexpr int kDm2Max = 8;    // transitions per DM2 cell held in registers
+constexpr int kRecLds = 128;  // records staged in LDS by the 3-D main kernel
 
 // Per-sim control block in device memory: everything a step reads that
 // changes between steps, so one captured step graph can be replayed.
@@ -18,12 +19,27 @@ struct Ctl {
   int32_t pad_;
 };
 
+constexpr int kPad = 32;  // elements before row 0 of every device array
+
+// Device layout of a component: logical (i,j,k) of a component with shape
+// (s0,s1,s2) lives at element kPad + (i*s1 + j)*p + k - koff.  The row
+// pitch p is s2 rounded up to 32 elements when s2 >= 32 (else s2), and
+// koff = 1 for Hx and Hy on a padded layout, so that the first updated
+// cell of every row (k = 0 for E and Hz, k = 1 for Hx, Hy) starts a
+// 128-byte line.  Material ids and CPML/UPML state use the same layout.
 template <class T>
 struct FieldArgs {
   T* f[6];
   int s[6][3];
+  int p[6];
+  int koff[6];
   unsigned active;
 };
+
+template <class T>
+__host__ __device__ __forceinline__ int didx(const FieldArgs<T>& a, int c, int i, int j, int k) {
+  return kPad + (i * a.s[c][1] + j) * a.p[c] + k - a.koff[c];
+}
 
 template <class T>
 struct MainArgs {
@@ -34,8 +50,11 @@ struct MainArgs {
   const R* prof;
   const int64_t* prof_off;
   void* pml[6];
-  int ncell[6];
+  int dsize[6];  // device elements per component (offset of the 2nd CPML word)
   R d[3];
+  R inv_d[3];
+  int nrec;
+  int n[3];  // nx, ny, nz
 };
 
 template <class T>
@@ -92,6 +111,8 @@ struct GhostArgs {
 
 template <class T>
 void launch_main(bool electric, const MainArgs<T>& m, int ext[3], int bx, int by, hipStream_t st);
+template <class T>
+void launch_main3d(bool electric, const MainArgs<T>& m, int bx, int by, int ichunk, hipStream_t st);
 template <class T>
 void launch_ade(int cls, int c, const AdeArgs<T>& m, hipStream_t st);
 template <class T>

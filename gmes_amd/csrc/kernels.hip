@@ -54,7 +54,7 @@ template <class T>
 __device__ __forceinline__ T ld(const FieldArgs<T>& a, int c, int i, int j, int k) {
   // an inactive component is (1,1,1) and reads as zero (src/cpp23_support.hh:52-74)
   if (!(a.active & (1u << c))) return vzero<T>();
-  return a.f[c][(i * a.s[c][1] + j) * a.s[c][2] + k];
+  return a.f[c][didx(a, c, i, j, k)];
 }
 
 // Curl differences D1, D2 of component C at (i,j,k): src/pw_dielectric.hh
@@ -91,26 +91,101 @@ template <> struct Ax<4> { static constexpr int a1 = 2, a2 = 0, a0 = 1; };
 template <> struct Ax<5> { static constexpr int a1 = 0, a2 = 1, a0 = 2; };
 
 // ------------------------------------------------------------- main update
-// One thread per (i,j,k); every component of the field type whose storage
-// contains (i,j,k) is updated.  ADE classes are skipped here.
+__device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ cplx<float> fma_(float a, cplx<float> b, cplx<float> c) {
+  return {__builtin_fmaf(a, b.re, c.re), __builtin_fmaf(a, b.im, c.im)};
+}
+
+// CPML update of one cell in registers (src/pw_cpml.hh:127-132,339-344):
+// psi1, psi2 are updated in place, the new field value is returned.  FP32
+// uses reciprocals (the FP32 profile holds 1/kappa) and FMAs; FP64 keeps
+// the reference's operation order.
 template <int C, class T>
-__device__ __forceinline__ void main_cell(const MainArgs<T>& m, int i, int j, int k) {
+__device__ __forceinline__ T cpml1(const MainArgs<T>& m, typename real_of<T>::type c0,
+                                   const typename real_of<T>::type* p1, const typename real_of<T>::type* p2,
+                                   T f, T& psi1, T& psi2, T D1, T D2) {
   using R = typename real_of<T>::type;
-  const FieldArgs<T>& a = m.fa;
-  const int flat = (i * a.s[C][1] + j) * a.s[C][2] + k;
-  const DevRec<R>& rec = m.recs[m.ids[C][flat]];
+  constexpr bool E = C < 3;
+  if constexpr (std::is_same<R, float>::value) {
+    const T a = vscl(m.inv_d[Ax<C>::a1], D1), b = vscl(m.inv_d[Ax<C>::a2], D2);
+    psi1 = fma_(p1[1], a, vscl(p1[0], psi1));
+    psi2 = fma_(p2[1], b, vscl(p2[0], psi2));
+    const T t = vsub(vadd(fma_(-p2[2], b, vscl(p1[2], a)), psi1), psi2);
+    return E ? fma_(c0, t, f) : fma_(-c0, t, f);
+  } else {
+    const R d1 = m.d[Ax<C>::a1], d2 = m.d[Ax<C>::a2];
+    psi1 = vadd(vscl(p1[0], psi1), vdiv(vscl(p1[1], D1), d1));
+    psi2 = vadd(vscl(p2[0], psi2), vdiv(vscl(p2[1], D2), d2));
+    const T t = vsub(vadd(vsub(vdiv(vdiv(D1, d1), p1[2]), vdiv(vdiv(D2, d2), p2[2])), psi1), psi2);
+    return E ? vadd(f, vscl(c0, t)) : vsub(f, vscl(c0, t));
+  }
+}
+
+// FP32 form of the Dielectric / CPML / UPML updates: same recursions, with
+// divisions replaced by products with reciprocals the host computed in FP64
+// (profiles hold 1/kappa for CPML; the record holds 1/eps).
+template <int C, class T>
+__device__ __forceinline__ void update_cell_f32(const MainArgs<T>& m, const DevRec<float>& rec, T* F, T f,
+                                                int i, int j, int k, T D1, T D2) {
+  constexpr bool E = C < 3;
+  const int cls = rec.cls;
+  const float id1 = m.inv_d[Ax<C>::a1], id2 = m.inv_d[Ax<C>::a2];
+  const T a = vscl(id1, D1), b = vscl(id2, D2);  // D1/d1, D2/d2
+  if (cls == GMES_CLS_DIELECTRIC) {
+    if constexpr (E) f = fma_(rec.c0, vsub(a, b), f);
+    else f = fma_(rec.c0, vsub(b, a), f);
+    *F = f;
+    return;
+  }
+  const int ix[3] = {i, j, k};
+  const int64_t* po = m.prof_off + (rec.prof * 6 + C) * 3;
+  const int flat = (int)(F - m.fa.f[C]);
+  T* st = static_cast<T*>(m.pml[C]);
+  if (cls == GMES_CLS_CPML) {
+    const float* p1 = m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 3;
+    const float* p2 = m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 3;
+    T psi1 = fma_(p1[1], a, vscl(p1[0], st[flat]));
+    T psi2 = fma_(p2[1], b, vscl(p2[0], st[m.dsize[C] + flat]));
+    st[flat] = psi1;
+    st[m.dsize[C] + flat] = psi2;
+    const T t = vsub(vadd(fma_(-p2[2], b, vscl(p1[2], a)), psi1), psi2);
+    if constexpr (E) f = fma_(rec.c0, t, f);
+    else f = fma_(-rec.c0, t, f);
+  } else {  // UPML
+    const float* p1 = m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 6;
+    const float* p2 = m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 6;
+    const float* p0 = m.prof + po[Ax<C>::a0] + ix[Ax<C>::a0] * 6;
+    const T dstore = st[flat];
+    const T cu = vsub(a, b);
+    const T dn = E ? fma_(p1[1], cu, vscl(p1[0], dstore)) : fma_(-p1[1], cu, vscl(p1[0], dstore));
+    st[flat] = dn;
+    f = fma_(p2[3] * rec.inv_eps, fma_(-p0[5], dstore, vscl(p0[4], dn)), vscl(p2[2], f));
+  }
+  *F = f;
+}
+
+// Update of one cell of component C given its curl differences D1, D2:
+// Dummy, Const, Dielectric, CPML, UPML.  ADE classes are skipped here.
+template <int C, class T>
+__device__ __forceinline__ void update_cell(const MainArgs<T>& m, const DevRec<typename real_of<T>::type>& rec,
+                                            int flat, int i, int j, int k, T D1, T D2) {
+  using R = typename real_of<T>::type;
   const int cls = rec.cls;
   constexpr bool E = C < 3;
   if (cls == GMES_CLS_DUMMY || cls >= GMES_CLS_DRUDE) return;
-  T* F = a.f[C] + flat;
+  T* F = m.fa.f[C] + didx(m.fa, C, i, j, k);
   if (cls == GMES_CLS_CONST) {  // src/pw_const.hh:96,198
     *F = make_val<T>(rec.val_re, rec.val_im);
     return;
   }
-  T D1, D2;
-  curl_terms<C>(a, i, j, k, D1, D2);
   const R d1 = m.d[Ax<C>::a1], d2 = m.d[Ax<C>::a2];
   T f = *F;
+  if constexpr (std::is_same<R, float>::value) {
+    // FP32: reciprocals precomputed in FP64 (1/d, 1/kappa, 1/eps) and
+    // explicit FMAs; FP32 runs are checked against FP64 to 1e-5.
+    update_cell_f32<C>(m, rec, F, f, i, j, k, D1, D2);
+    return;
+  }
   if (cls == GMES_CLS_DIELECTRIC) {  // src/pw_dielectric.hh:118,292
     if constexpr (E) f = vadd(f, vscl(rec.c0, vsub(vdiv(D1, d1), vdiv(D2, d2))));
     else f = vadd(f, vscl(rec.c0, vsub(vdiv(D2, d2), vdiv(D1, d1))));
@@ -121,11 +196,11 @@ __device__ __forceinline__ void main_cell(const MainArgs<T>& m, int i, int j, in
     if (cls == GMES_CLS_CPML) {  // src/pw_cpml.hh:127-132,339-344
       const R* p1 = m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 3;
       const R* p2 = m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 3;
-      T psi1 = st[flat], psi2 = st[m.ncell[C] + flat];
+      T psi1 = st[flat], psi2 = st[m.dsize[C] + flat];
       psi1 = vadd(vscl(p1[0], psi1), vdiv(vscl(p1[1], D1), d1));
       psi2 = vadd(vscl(p2[0], psi2), vdiv(vscl(p2[1], D2), d2));
       st[flat] = psi1;
-      st[m.ncell[C] + flat] = psi2;
+      st[m.dsize[C] + flat] = psi2;
       T t = vsub(vadd(vsub(vdiv(vdiv(D1, d1), p1[2]), vdiv(vdiv(D2, d2), p2[2])), psi1), psi2);
       if constexpr (E) f = vadd(f, vscl(rec.c0, t));
       else f = vsub(f, vscl(rec.c0, t));
@@ -144,6 +219,19 @@ __device__ __forceinline__ void main_cell(const MainArgs<T>& m, int i, int j, in
   *F = f;
 }
 
+template <int C, class T>
+__device__ __forceinline__ void main_cell(const MainArgs<T>& m, int i, int j, int k) {
+  const FieldArgs<T>& a = m.fa;
+  const int flat = didx(a, C, i, j, k);
+  const auto& rec = m.recs[m.ids[C][flat]];
+  if (rec.cls == GMES_CLS_DUMMY || rec.cls >= GMES_CLS_DRUDE) return;
+  T D1 = vzero<T>(), D2 = vzero<T>();
+  if (rec.cls != GMES_CLS_CONST) curl_terms<C>(a, i, j, k, D1, D2);
+  update_cell<C>(m, rec, flat, i, j, k, D1, D2);
+}
+
+// Generic main kernel (any mode): one thread per (i,j,k); every component
+// of the field type whose storage contains (i,j,k) is updated.
 template <bool E, class T>
 __global__ void __launch_bounds__(256) k_main(MainArgs<T> m) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -158,6 +246,84 @@ __global__ void __launch_bounds__(256) k_main(MainArgs<T> m) {
   GMES_MAIN_COMP(C0 + 1)
   GMES_MAIN_COMP(C0 + 2)
 #undef GMES_MAIN_COMP
+}
+
+// 3-D main kernel (all six components active).  Thread (i,j,k) owns the
+// non-ghost cell of every component of the field type: for E the cells
+// (i,j,k) of Ex, Ey, Ez; for H the cells Hx(i,j+1,k+1), Hy(i+1,j,k+1),
+// Hz(i+1,j+1,k) -- the ghost planes are never updated, so the thread grid
+// is exactly nx*ny*nz.  A block covers a (j,k) tile and marches along i
+// through `ichunk` planes, keeping the plane it will need again (H at i-1
+// for E, E at i+1 for H) in registers; each curl operand is loaded once and
+// shared by the components that use it.  The record table is staged in LDS.
+template <bool E, class T>
+__global__ void __launch_bounds__(256) k_main3d(MainArgs<T> m, int ichunk) {
+  using R = typename real_of<T>::type;
+  __shared__ DevRec<R> srec[kRecLds];
+  const DevRec<R>* recs = m.recs;
+  if (m.nrec <= kRecLds) {
+    for (int t = threadIdx.y * blockDim.x + threadIdx.x; t < m.nrec; t += blockDim.x * blockDim.y)
+      srec[t] = m.recs[t];
+    __syncthreads();
+    recs = srec;
+  }
+  const int nx = m.n[0], ny = m.n[1], nz = m.n[2];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (k >= nz || j >= ny) return;
+  const int i0 = blockIdx.z * ichunk;
+  const int i1 = min(i0 + ichunk, nx);
+  T* const* f = m.fa.f;
+  const FieldArgs<T>& a = m.fa;
+  // per component: element of (i,j,k) = B + i*P + j*W + k (pitched layout)
+  int B[6], P[6], W[6];
+  for (int c = 0; c < 6; ++c) { B[c] = kPad - a.koff[c]; W[c] = a.p[c]; P[c] = a.s[c][1] * a.p[c]; }
+  if constexpr (E) {
+    const T* hx = f[3];
+    const T* hy = f[4];
+    const T* hz = f[5];
+    T hz_prev = hz[B[5] + i0 * P[5] + (j + 1) * W[5] + k];  // H'z(i0-1,j,k) = Hz[i0, j+1, k]
+    T hy_prev = hy[B[4] + i0 * P[4] + j * W[4] + k + 1];    // H'y(i0-1,j,k) = Hy[i0, j, k+1]
+    for (int i = i0; i < i1; ++i) {
+      const T hxc = hx[B[3] + i * P[3] + (j + 1) * W[3] + k + 1];
+      const T hx_jm = hx[B[3] + i * P[3] + j * W[3] + k + 1];
+      const T hx_km = hx[B[3] + i * P[3] + (j + 1) * W[3] + k];
+      const T hyc = hy[B[4] + (i + 1) * P[4] + j * W[4] + k + 1];
+      const T hy_km = hy[B[4] + (i + 1) * P[4] + j * W[4] + k];
+      const T hzc = hz[B[5] + (i + 1) * P[5] + (j + 1) * W[5] + k];
+      const T hz_jm = hz[B[5] + (i + 1) * P[5] + j * W[5] + k];
+      const int fx = B[0] + i * P[0] + j * W[0] + k, fy = B[1] + i * P[1] + j * W[1] + k;
+      const int fz = B[2] + i * P[2] + j * W[2] + k;
+      update_cell<0>(m, recs[m.ids[0][fx]], fx, i, j, k, vsub(hzc, hz_jm), vsub(hyc, hy_km));
+      update_cell<1>(m, recs[m.ids[1][fy]], fy, i, j, k, vsub(hxc, hx_km), vsub(hzc, hz_prev));
+      update_cell<2>(m, recs[m.ids[2][fz]], fz, i, j, k, vsub(hyc, hy_prev), vsub(hxc, hx_jm));
+      hz_prev = hzc;
+      hy_prev = hyc;
+    }
+  } else {
+    const T* ex = f[0];
+    const T* ey = f[1];
+    const T* ez = f[2];
+    T ey_cur = ey[B[1] + i0 * P[1] + j * W[1] + k];
+    T ez_cur = ez[B[2] + i0 * P[2] + j * W[2] + k];
+    for (int i = i0; i < i1; ++i) {
+      const T exc = ex[B[0] + i * P[0] + j * W[0] + k];
+      const T ex_kp = ex[B[0] + i * P[0] + j * W[0] + k + 1];
+      const T ex_jp = ex[B[0] + i * P[0] + (j + 1) * W[0] + k];
+      const T ey_kp = ey[B[1] + i * P[1] + j * W[1] + k + 1];
+      const T ey_ip = ey[B[1] + (i + 1) * P[1] + j * W[1] + k];
+      const T ez_jp = ez[B[2] + i * P[2] + (j + 1) * W[2] + k];
+      const T ez_ip = ez[B[2] + (i + 1) * P[2] + j * W[2] + k];
+      const int fx = B[3] + i * P[3] + (j + 1) * W[3] + k + 1;    // Hx(i, j+1, k+1)
+      const int fy = B[4] + (i + 1) * P[4] + j * W[4] + k + 1;    // Hy(i+1, j, k+1)
+      const int fz = B[5] + (i + 1) * P[5] + (j + 1) * W[5] + k;  // Hz(i+1, j+1, k)
+      update_cell<3>(m, recs[m.ids[3][fx]], fx, i, j + 1, k + 1, vsub(ez_jp, ez_cur), vsub(ey_kp, ey_cur));
+      update_cell<4>(m, recs[m.ids[4][fy]], fy, i + 1, j, k + 1, vsub(ex_kp, exc), vsub(ez_ip, ez_cur));
+      update_cell<5>(m, recs[m.ids[5][fz]], fz, i + 1, j + 1, k, vsub(ey_ip, ey_cur), vsub(ex_jp, exc));
+      ey_cur = ey_ip;
+      ez_cur = ez_ip;
+    }
+  }
 }
 
 // ----------------------------------------------------------------- ADE kernels
@@ -184,7 +350,7 @@ __global__ void __launch_bounds__(256) k_ade_drude(AdeArgs<T> m) {
   const int P = rec.n1;
   const R* cc = a + 3 * P;
   T* st = static_cast<T*>(m.state);
-  T* F = m.fa.f[C] + flat;
+  T* F = m.fa.f[C] + didx(m.fa, C, i, j, k);
   const T e_now = *F;
   T sum = vzero<T>();
   for (int p = 0; p < P; ++p) {
@@ -219,7 +385,7 @@ __global__ void __launch_bounds__(256) k_ade_dcp(AdeArgs<T> m) {
   T* st = static_cast<T*>(m.state);
   const int n = m.n;
   // words: 0 e_old | 1..D q_old | D+1..2D q_now | 2D+1.. p_old | 2D+C+1.. p_now
-  T* F = m.fa.f[C] + flat;
+  T* F = m.fa.f[C] + didx(m.fa, C, i, j, k);
   const T e_now = *F;
   const T e_old = st[q];
   T ds = vzero<T>(), cs = vzero<T>();
@@ -271,7 +437,7 @@ __global__ void __launch_bounds__(256) k_ade_plrc(AdeArgs<T> m) {
   const R* cc = b + 6 * NC;
   Z* st = static_cast<Z*>(m.state);
   const int n = m.n;
-  T* F = m.fa.f[C] + flat;
+  T* F = m.fa.f[C] + didx(m.fa, C, i, j, k);
   const T fv = *F;
   const Z e_now = {re_of(fv), im_of(fv)};
   R sdr = R(0), sdi = R(0);
@@ -351,7 +517,7 @@ __global__ void __launch_bounds__(256) k_ade_dm2(AdeArgs<T> m) {
     const R dd = R(2) / hbar * gamma * rho30 * exp(t / t2);
     R* st = static_cast<R*>(m.state);
     for (int p = 0; p < 3 * N; ++p) { u[p] = st[p * m.n + q]; u_new[p] = u[p]; }
-    T* F = m.fa.f[C] + flat;
+    T* F = m.fa.f[C] + didx(m.fa, C, i, j, k);
     R e_new = *F;
     const R e_old = *F;
     T D1, D2;
@@ -463,7 +629,7 @@ __global__ void __launch_bounds__(256) k_ghost(GhostArgs<T> m) {
     int sx[3] = {ix[0], ix[1], ix[2]};
     sx[a1] = src1;
     if (ix[a2] == dst2) sx[a2] = src2;
-    v = f[(sx[0] * s[1] + sx[1]) * s[2] + sx[2]];
+    v = f[didx(m.fa, c, sx[0], sx[1], sx[2])];
     if constexpr (is_cplx<T>::value) {
       v = cmul(m.ph[t][0], v);
       if (ix[a2] == dst2) v = cmul(m.ph[t][1], v);
@@ -477,10 +643,10 @@ __global__ void __launch_bounds__(256) k_ghost(GhostArgs<T> m) {
     ix[a2] = dst2;
     int sx[3] = {ix[0], ix[1], ix[2]};
     sx[a2] = src2;
-    v = f[(sx[0] * s[1] + sx[1]) * s[2] + sx[2]];
+    v = f[didx(m.fa, c, sx[0], sx[1], sx[2])];
     if constexpr (is_cplx<T>::value) v = cmul(m.ph[t][1], v);
   }
-  f[(ix[0] * s[1] + ix[1]) * s[2] + ix[2]] = v;
+  f[didx(m.fa, c, ix[0], ix[1], ix[2])] = v;
 }
 
 // --------------------------------------------------------------- launchers
@@ -490,6 +656,14 @@ void launch_main(bool electric, const MainArgs<T>& m, int ext[3], int bx, int by
   dim3 grid((ext[2] + bx - 1) / bx, (ext[1] + by - 1) / by, ext[0]);
   if (electric) hipLaunchKernelGGL((k_main<true, T>), grid, block, 0, st, m);
   else hipLaunchKernelGGL((k_main<false, T>), grid, block, 0, st, m);
+}
+
+template <class T>
+void launch_main3d(bool electric, const MainArgs<T>& m, int bx, int by, int ichunk, hipStream_t st) {
+  dim3 block(bx, by, 1);
+  dim3 grid((m.n[2] + bx - 1) / bx, (m.n[1] + by - 1) / by, (m.n[0] + ichunk - 1) / ichunk);
+  if (electric) hipLaunchKernelGGL((k_main3d<true, T>), grid, block, 0, st, m, ichunk);
+  else hipLaunchKernelGGL((k_main3d<false, T>), grid, block, 0, st, m, ichunk);
 }
 
 template <class T>
@@ -526,6 +700,7 @@ void launch_ghost(const GhostArgs<T>& m, int64_t total, hipStream_t st) {
 
 #define GMES_INSTANTIATE(T)                                                                     \
   template void launch_main<T>(bool, const MainArgs<T>&, int[3], int, int, hipStream_t);       \
+  template void launch_main3d<T>(bool, const MainArgs<T>&, int, int, int, hipStream_t);        \
   template void launch_ade<T>(int, int, const AdeArgs<T>&, hipStream_t);                       \
   template void launch_src<T>(const SrcArgs<T>&, hipStream_t);                                  \
   template void launch_probe<T>(const ProbeArgs<T>&, hipStream_t);                              \
