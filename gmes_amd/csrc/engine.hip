@@ -388,7 +388,16 @@ void enqueue_step(gmes_sim* s, hipStream_t st) {
 
 void advance_host_clock(gmes_sim* s, int64_t n) {
   s->step += n;
+  s->probe_n = s->step - s->probe_base;
   for (int i = 0; i < s->naux; ++i) advance_host_clock(s->aux[i], n);
+}
+
+// Waits for all work that may touch s: its own stream and, for an aux sim,
+// the stream of the sim that advances it.
+hipError_t sync_exec(gmes_sim* s) {
+  hipError_t e = hipStreamSynchronize(s->stream);
+  if (e == hipSuccess && s->parent) e = hipStreamSynchronize(s->parent->stream);
+  return e;
 }
 
 // Checks that every step in [s->step, s->step + n) is covered by the
@@ -396,7 +405,7 @@ void advance_host_clock(gmes_sim* s, int64_t n) {
 int check_windows(gmes_sim* s, int64_t n) {
   bool waves = false;
   for (const auto& o : s->ops)
-    if (o.kind == GMES_SRC_SET || o.kind == GMES_SRC_SUB) waves = true;
+    if (o.kind == GMES_SRC_SET || o.kind == GMES_SRC_SUB || o.wave >= 0) waves = true;
   if (waves && (s->step < s->wave_step0 || s->step + n > s->wave_step0 + s->wave_n))
     return set_error(GMES_EINVAL, "waveforms cover steps [%lld, %lld), need [%lld, %lld)",
                      (long long)s->wave_step0, (long long)(s->wave_step0 + s->wave_n),
@@ -409,6 +418,10 @@ int check_windows(gmes_sim* s, int64_t n) {
 }
 
 int ensure_probe_capacity(gmes_sim* s, int64_t n) {
+  for (int i = 0; i < s->naux; ++i) {
+    int rc = ensure_probe_capacity(s->aux[i], n);
+    if (rc) return rc;
+  }
   if (s->probes.empty()) return GMES_OK;
   const int64_t need = (s->step - s->probe_base) + n;
   if (need <= s->probe_cap) return GMES_OK;
@@ -417,7 +430,7 @@ int ensure_probe_capacity(gmes_sim* s, int64_t n) {
   double* nb = nullptr;
   int rc = dev_alloc(s, (void**)&nb, (size_t)cap * s->probes.size() * 2 * sizeof(double));
   if (rc) return rc;
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(sync_exec(s));
   if (s->d_probe_buf) {
     const int64_t used = s->step - s->probe_base;
     for (size_t p = 0; p < s->probes.size(); ++p)
@@ -428,6 +441,7 @@ int ensure_probe_capacity(gmes_sim* s, int64_t n) {
   s->d_probe_buf = nb;
   s->probe_cap = cap;
   s->graph_dirty = true;
+  if (s->parent) s->parent->graph_dirty = true;
   return GMES_OK;
 }
 
@@ -437,6 +451,8 @@ void destroy_graph(gmes_sim* s) {
     s->graph_exec = nullptr;
   }
   s->graph_dirty = true;
+  // the parent's graph holds this aux sim's kernels and their arguments
+  if (s->parent) destroy_graph(s->parent);
 }
 
 int build_graph(gmes_sim* s) {
@@ -491,13 +507,14 @@ int do_step(gmes_sim* s, int64_t n) {
 
 int ctl_write(gmes_sim* s) {
   Ctl c;
+  HIP_TRY(sync_exec(s));
   HIP_TRY(hipMemcpyAsync(&c, s->d_clock, sizeof c, hipMemcpyDeviceToHost, s->stream));
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(sync_exec(s));
   c.wave_step0 = s->wave_step0;
   c.wave_n = s->wave_n;
   c.probe_base = s->probe_base;
   HIP_TRY(hipMemcpyAsync(s->d_clock, &c, sizeof c, hipMemcpyHostToDevice, s->stream));
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(sync_exec(s));
   return GMES_OK;
 }
 
@@ -571,6 +588,10 @@ int gmes_sim_destroy(gmes_sim* s) {
   if (!s) return GMES_OK;
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  // an aux sim is advanced on its parent's stream: drain that too
+  if (s->parent && s->parent->stream) (void)hipStreamSynchronize(s->parent->stream);
+  for (int i = 0; i < s->naux; ++i)
+    if (s->aux[i]) s->aux[i]->parent = nullptr;
   destroy_graph(s);
   collect_timing(s);
   for (int c = 0; c < 6; ++c) {
@@ -665,7 +686,7 @@ int gmes_sim_set_profile(gmes_sim* s, int32_t p, int32_t c, int32_t axis, const 
 int gmes_sim_commit(gmes_sim* s) {
   if (s->recs.empty()) return set_error(GMES_EINVAL, "materials not set");
   HIP_TRY(hipSetDevice(s->device));
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(sync_exec(s));
   free_ade(s);
   for (int c = 0; c < 6; ++c) {
     for (int k = 0; k < GMES_NCLS; ++k) s->has_cls[c][k] = false;
@@ -750,6 +771,7 @@ int gmes_sim_set_sources(gmes_sim* s, const gmes_src_op* ops, int64_t nops, int3
       if (o.wave < 0 || o.wave >= nwaves) return set_error(GMES_EINVAL, "bad waveform row");
     } else if (o.kind == GMES_SRC_AUX_ADD || o.kind == GMES_SRC_AUX_SUB) {
       if (o.aux < 0 || o.aux >= s->naux) return set_error(GMES_EINVAL, "auxiliary grid %d not attached", o.aux);
+      if (o.wave < -1 || o.wave >= nwaves) return set_error(GMES_EINVAL, "bad envelope row");
       if (o.aux_comp < 0 || o.aux_comp > 5) return set_error(GMES_EINVAL, "bad aux component");
       const gmes_sim* a = s->aux[o.aux];
       if (!((a->g.active >> o.aux_comp) & 1u)) return set_error(GMES_EINVAL, "inactive aux component");
@@ -760,7 +782,7 @@ int gmes_sim_set_sources(gmes_sim* s, const gmes_src_op* ops, int64_t nops, int3
     }
   }
   HIP_TRY(hipSetDevice(s->device));
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(sync_exec(s));
   s->ops.assign(ops, ops + nops);
   s->nwaves = nwaves;
   int rc = s->fp64 ? upload_sources<double>(s) : upload_sources<float>(s);
@@ -772,7 +794,7 @@ int gmes_sim_set_waveforms(gmes_sim* s, int64_t step0, int64_t nsteps, const dou
   if (nsteps < 0) return set_error(GMES_EINVAL, "negative step count");
   HIP_TRY(hipSetDevice(s->device));
   const int64_t need = std::max<int64_t>(1, (int64_t)s->nwaves * nsteps * 2);
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(sync_exec(s));
   if (need > s->wave_cap) {
     if (s->d_waves) dev_free(s, s->d_waves, (size_t)s->wave_cap * sizeof(double));
     s->d_waves = nullptr;
@@ -794,8 +816,9 @@ int gmes_sim_attach_aux(gmes_sim* s, gmes_sim* aux, int32_t* slot) {
   if (aux->g.dtype != s->g.dtype) return set_error(GMES_ETYPE, "aux dtype mismatch");
   if (aux->device != s->device) return set_error(GMES_EINVAL, "aux sim on another device");
   if (aux->naux) return set_error(GMES_EINVAL, "an auxiliary grid cannot have its own");
-  if (aux->step != s->step) return set_error(GMES_EINVAL, "aux sim is at another step");
+  if (aux->parent) return set_error(GMES_EINVAL, "auxiliary grid already attached");
   s->aux[s->naux] = aux;
+  aux->parent = s;
   void* ptrs[6];
   for (int c = 0; c < 6; ++c) ptrs[c] = aux->f[c];
   HIP_TRY(hipSetDevice(s->device));
@@ -810,7 +833,7 @@ int gmes_sim_add_probe(gmes_sim* s, int32_t c, int64_t cell, int32_t* id) {
     return set_error(GMES_ERANGE, "probe index out of bounds");
   if (s->step != s->probe_base) return set_error(GMES_EINVAL, "read pending probe values first");
   HIP_TRY(hipSetDevice(s->device));
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(sync_exec(s));
   s->probes.push_back({c, cell});
   std::vector<int64_t> cells;
   std::vector<int32_t> comps;
@@ -840,7 +863,7 @@ static std::vector<int64_t>& probe_cursor(gmes_sim* s) {
 int gmes_sim_read_probe(gmes_sim* s, int32_t id, double* out, int64_t max, int64_t* n) {
   if (id < 0 || id >= (int32_t)s->probes.size()) return set_error(GMES_EINVAL, "bad probe id");
   HIP_TRY(hipSetDevice(s->device));
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(sync_exec(s));
   std::vector<int64_t>& cur = probe_cursor(s);
   cur.resize(s->probes.size(), 0);
   const int64_t avail = s->probe_n - cur[id];
@@ -869,13 +892,13 @@ int gmes_sim_write_field(gmes_sim* s, int32_t c, const void* host, int64_t bytes
   if (s->pitch[c] == s->shape[c][2] && s->koff[c] == 0) {
     HIP_TRY(hipMemcpyAsync(static_cast<char*>(s->f[c]) + (size_t)kPad * s->esize, host, (size_t)bytes,
                            hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(sync_exec(s));
     return GMES_OK;
   }
   HIP_TRY(hipMemcpy2DAsync(static_cast<char*>(s->f[c]) + (size_t)(kPad - s->koff[c]) * s->esize,
                            (size_t)s->pitch[c] * s->esize, host, row, row,
                            (size_t)(s->shape[c][0] * s->shape[c][1]), hipMemcpyHostToDevice, s->stream));
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(sync_exec(s));
   return GMES_OK;
 }
 
@@ -887,13 +910,13 @@ int gmes_sim_read_field(gmes_sim* s, int32_t c, void* host, int64_t bytes) {
   if (s->pitch[c] == s->shape[c][2] && s->koff[c] == 0) {
     HIP_TRY(hipMemcpyAsync(host, static_cast<const char*>(s->f[c]) + (size_t)kPad * s->esize, (size_t)bytes,
                            hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(sync_exec(s));
     return GMES_OK;
   }
   HIP_TRY(hipMemcpy2DAsync(host, row, static_cast<const char*>(s->f[c]) + (size_t)(kPad - s->koff[c]) * s->esize,
                            (size_t)s->pitch[c] * s->esize, row, (size_t)(s->shape[c][0] * s->shape[c][1]),
                            hipMemcpyDeviceToHost, s->stream));
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(sync_exec(s));
   return GMES_OK;
 }
 
@@ -924,7 +947,7 @@ int gmes_sim_snapshot(gmes_sim* s, void* host, int64_t bytes) {
   for (auto& x : b) t += (int64_t)x.second;
   if (t != bytes) return set_error(GMES_EINVAL, "size mismatch");
   HIP_TRY(hipSetDevice(s->device));
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(sync_exec(s));
   char* p = static_cast<char*>(host);
   for (auto& x : b) { HIP_TRY(hipMemcpy(p, x.first, x.second, hipMemcpyDeviceToHost)); p += x.second; }
   return GMES_OK;
@@ -937,7 +960,7 @@ int gmes_sim_restore(gmes_sim* s, const void* host, int64_t bytes) {
   for (auto& x : b) t += (int64_t)x.second;
   if (t != bytes) return set_error(GMES_EINVAL, "size mismatch");
   HIP_TRY(hipSetDevice(s->device));
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(sync_exec(s));
   const char* p = static_cast<const char*>(host);
   for (auto& x : b) { HIP_TRY(hipMemcpy(x.first, p, x.second, hipMemcpyHostToDevice)); p += x.second; }
   return GMES_OK;
@@ -947,7 +970,7 @@ int gmes_sim_step(gmes_sim* s, int64_t nsteps) { return do_step(s, nsteps); }
 
 int gmes_sim_sync(gmes_sim* s) {
   HIP_TRY(hipSetDevice(s->device));
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(sync_exec(s));
   Ctl c;
   HIP_TRY(hipMemcpy(&c, s->d_clock, sizeof c, hipMemcpyDeviceToHost));
   if (c.err == 1) return set_error(GMES_ERUNTIME, "Dm2 corrector produced an invalid error");
@@ -984,7 +1007,7 @@ int gmes_sim_set_tile(gmes_sim* s, int32_t bx, int32_t by) {
 
 int gmes_sim_profile(gmes_sim* s, int32_t enable) {
   HIP_TRY(hipSetDevice(s->device));
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(sync_exec(s));
   collect_timing(s);
   s->timing.on = enable != 0;
   for (int i = 0; i < 6; ++i) { s->timing.ms[i] = 0; s->timing.launches[i] = 0; }
@@ -993,7 +1016,7 @@ int gmes_sim_profile(gmes_sim* s, int32_t enable) {
 
 int gmes_sim_read_profile(gmes_sim* s, double ms[6], int64_t launches[6]) {
   HIP_TRY(hipSetDevice(s->device));
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(sync_exec(s));
   int rc = collect_timing(s);
   if (rc) return rc;
   for (int i = 0; i < 6; ++i) { ms[i] = s->timing.ms[i]; launches[i] = s->timing.launches[i]; }

@@ -109,6 +109,7 @@ struct gmes_sim {
   // aux sims
   gmes_sim* aux[8] = {};
   int32_t naux = 0;
+  gmes_sim* parent = nullptr;  // the sim whose stream advances this aux sim
   void* d_aux_ptrs = nullptr;  // [8][6] device pointers of aux fields
   // probes
   struct Probe { int32_t comp; int64_t cell; };

@@ -577,7 +577,13 @@ __global__ void __launch_bounds__(256) k_src(SrcArgs<T> m) {
       v = op.kind == GMES_SRC_SET ? wv : vsub(v, wv);
     } else {
       const T* aux = static_cast<const T*>(m.aux[op.aux * 6 + op.aux_comp]);
-      const T inc = vadd(vscl(op.r0, aux[op.i0]), vscl(op.r1, aux[op.i1]));
+      T a0 = aux[op.i0], a1 = aux[op.i1];
+      if (op.wave >= 0) {  // Gaussian-beam envelope scales each sample first
+        const R env = (R)m.waves[(op.wave * m.ctl->wave_n + row) * 2];
+        a0 = vscl(env, a0);
+        a1 = vscl(env, a1);
+      }
+      const T inc = vadd(vscl(op.r0, a0), vscl(op.r1, a1));
       v = op.kind == GMES_SRC_AUX_ADD ? vadd(v, vscl(op.k, inc)) : vsub(v, vscl(op.k, inc));
     }
   }

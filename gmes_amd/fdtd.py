@@ -142,6 +142,11 @@ class FDTD(object):
 
         self.src_list = [] if src_list is None else src_list
         for so in self.src_list:
+            if hasattr(so, "aux_fdtd"):
+                # the auxiliary grid runs on the same engine, device and precision
+                so._engine = _engine
+                so._precision = precision
+                so._device = self.device
             so.init(self.geom_tree, self.space, self.cmplx)
         self.pw_material = {}
         self.pw_source = {}
@@ -260,6 +265,7 @@ class FDTD(object):
         """Map point sources and TFSF faces to engine source operations, in
         the reference's order (gmes/fdtd.py:505-605, gmes/pw_source.py:18-46)."""
         self._waves = []
+        env_rows = {}
         ops = []
         for comp in tuple(self.e_field_compnt) + tuple(self.h_field_compnt):
             per_type = {}
@@ -297,6 +303,14 @@ class FDTD(object):
                         else:
                             for key in ("aux", "aux_comp", "i0", "i1", "k", "r0", "r1"):
                                 op[key] = payload[key]
+                            env = payload.get("envelope")
+                            if env is None:
+                                op["wave"] = -1
+                            else:
+                                if id(env) not in env_rows:
+                                    env_rows[id(env)] = len(self._waves)
+                                    self._waves.append(env)
+                                op["wave"] = env_rows[id(env)]
                         ops.append(op)
             self.pw_source[comp] = {t: PwMaterialView(t, len(tb)) for t, tb in per_type.items()}
         arr = np.array(ops, _abi.SRC_OP_DTYPE) if ops else np.zeros(0, _abi.SRC_OP_DTYPE)
@@ -576,6 +590,8 @@ class FDTD(object):
         if self._sim is not None and self._lib is not None:
             self._lib.sim_destroy(self._sim)
             self._sim = None
+        for aux in getattr(self, "_aux_fdtds", ()):
+            aux.close()
 
     def __del__(self):
         try:

@@ -9,13 +9,14 @@ runs at (n = k + 1/2 for E, n = k + 1 for H).
 """
 
 from cmath import exp as cexp
-from math import exp, pi, sin
+from copy import deepcopy
+from math import exp, floor, pi, sin, sqrt
 
 import numpy as np
 from numpy import inf
 
 from . import constant as const
-from ._abi import SRC_SET, SRC_SUB
+from ._abi import SRC_AUX_ADD, SRC_AUX_SUB, SRC_SET, SRC_SUB
 from .geometry import in_range
 
 
@@ -204,3 +205,322 @@ class PointSource(Src):
 
         kind = SRC_SET if hard else SRC_SUB
         return [(tuple(int(v) for v in idx), "point", kind, WaveRow(value, electric))]
+
+
+# ---------------------------------------------------------------------------
+# Total-field / scattered-field plane-wave source
+#
+# Face table (gmes/source.py:705-1406): for every updated component, the
+# faces in the reference's order, each with
+#   face         : the Directional of the face,
+#   axis, plus   : the face normal and whether it is the + side,
+#   cos_dir, cos : which unit vector is dotted with h_direction (E faces) or
+#                  e_direction (H faces) to give the cosine,
+#   idx_comp     : component whose space_to_<c>_index gives the index range,
+#   lo_off, hi_off: offsets added to that range's low and high ends,
+#   samp_comp, samp_off: the component/offset whose staggered position is
+#                  projected on the beam axis to sample the incident field,
+#   sign         : +1 for F += ..., -1 for F -= ... (gmes/pw_source.py:197-486).
+_X, _Y, _Z = (1, 0, 0), (0, 1, 0), (0, 0, 1)
+_C = const
+_TFSF_FACES = {
+    _C.Ex: [(_C.MinusY, 1, False, _Z, _C.Ex, (0, 0, 0), (0, 0, 0), _C.Hz, (1, 0, 0), -1),
+            (_C.PlusY, 1, True, _Z, _C.Ex, (0, 0, 0), (0, 0, 0), _C.Hz, (1, 1, 0), +1),
+            (_C.MinusZ, 2, False, _Y, _C.Ex, (0, 0, 0), (0, 0, 0), _C.Hy, (1, 0, 0), +1),
+            (_C.PlusZ, 2, True, _Y, _C.Ex, (0, 0, 0), (0, 0, 0), _C.Hy, (1, 0, 1), -1)],
+    _C.Ey: [(_C.MinusZ, 2, False, _X, _C.Ey, (0, 0, 0), (0, 0, 0), _C.Hx, (0, 1, 0), -1),
+            (_C.PlusZ, 2, True, _X, _C.Ey, (0, 0, 0), (0, 0, 0), _C.Hx, (0, 1, 1), +1),
+            (_C.MinusX, 0, False, _Z, _C.Ey, (0, 0, 0), (0, 0, 0), _C.Hz, (0, 1, 0), +1),
+            (_C.PlusX, 0, True, _Z, _C.Ey, (0, 0, 0), (0, 0, 0), _C.Hz, (1, 1, 0), -1)],
+    _C.Ez: [(_C.MinusX, 0, False, _Y, _C.Ez, (0, 0, 0), (0, 0, 0), _C.Hy, (0, 0, 1), -1),
+            (_C.PlusX, 0, True, _Y, _C.Ez, (0, 0, 0), (0, 0, 0), _C.Hy, (1, 0, 1), +1),
+            (_C.MinusY, 1, False, _X, _C.Ez, (0, 0, 0), (0, 0, 0), _C.Hx, (0, 0, 1), +1),
+            (_C.PlusY, 1, True, _X, _C.Ez, (0, 0, 0), (0, 0, 0), _C.Hx, (0, 1, 1), -1)],
+    _C.Hx: [(_C.MinusY, 1, False, _Z, _C.Ez, (0, 0, 1), (0, 0, 1), _C.Ez, (0, 0, -1), +1),
+            (_C.PlusY, 1, True, _Z, _C.Ez, (0, 1, 1), (0, 1, 1), _C.Ez, (0, -1, -1), -1),
+            (_C.MinusZ, 2, False, _Y, _C.Ey, (0, 1, 0), (0, 1, 0), _C.Ey, (0, -1, -1), -1),
+            (_C.PlusZ, 2, True, _Y, _C.Ey, (0, 1, 1), (0, 1, 1), _C.Ey, (0, -1, 0), +1)],
+    _C.Hy: [(_C.MinusZ, 2, False, _X, _C.Ex, (1, 0, 0), (1, 0, 0), _C.Ex, (-1, 0, 0), +1),
+            (_C.PlusZ, 2, True, _X, _C.Ex, (1, 0, 1), (1, 0, 1), _C.Ex, (-1, 0, -1), -1),
+            (_C.MinusX, 0, False, _Z, _C.Ez, (0, 0, 1), (0, 0, 1), _C.Ez, (0, 0, -1), -1),
+            (_C.PlusX, 0, True, _Z, _C.Ez, (1, 0, 1), (1, 0, 1), _C.Ez, (-1, 0, -1), +1)],
+    _C.Hz: [(_C.MinusX, 0, False, _Y, _C.Ey, (0, 1, 0), (0, 1, 0), _C.Ey, (0, -1, 0), +1),
+            (_C.PlusX, 0, True, _Y, _C.Ey, (1, 1, 0), (1, 1, 0), _C.Ey, (-1, -1, 0), -1),
+            (_C.MinusY, 1, False, _X, _C.Ex, (1, 0, 0), (1, 0, 0), _C.Ex, (0, -1, 0), -1),
+            (_C.PlusY, 1, True, _X, _C.Ex, (1, 1, 0), (1, 1, 0), _C.Ex, (-1, -1, 0), +1)],
+}
+
+
+class TotalFieldScatteredField(Src):
+    """Plane wave launched through a total-field/scattered-field box
+    (gmes/source.py:370-1522).  The incident field comes from a 1-D
+    auxiliary TEMz grid with matched numerical dispersion
+    (gmes/source.py:549-630) that runs on the device between the E and H
+    half steps of the main grid."""
+
+    def __init__(self, src_time, center, size, direction, polarization, amp=1):
+        if not isinstance(src_time, SrcTime):
+            raise TypeError("src_time must be an instance of SrcTime.")
+        self.src_time = src_time
+        self.k = np.array(direction, np.double) / np.linalg.norm(direction)
+        self.center = np.array(center, np.double)
+        if not np.isfinite(self.center).all():
+            raise ValueError("center must contain only finite values")
+        self.size = np.array(size, np.double)
+        if not np.isfinite(self.size).all():
+            raise ValueError("size must contain only finite values")
+        self.half_size = 0.5 * self.size
+        self.e_direction = np.array(polarization, np.double) / np.linalg.norm(polarization)
+        self.h_direction = np.cross(self.k, self.e_direction)
+        self.amp = float(amp)
+        self.on_axis_k = self._axis_in_k()
+        self.aux_fdtd = None
+        self._engine = None
+        self._precision = "fp64"
+        self._device = 0
+
+    def init(self, geom_tree, space, cmplx):
+        self.geom_tree = geom_tree
+        self.src_time.init(cmplx)
+        self.aux_fdtd = self._get_aux_fdtd(space, geom_tree, cmplx)
+        self.aux_fdtd.init()
+
+    def step(self):
+        pass  # the engine advances the auxiliary grid with the main one
+
+    def display_info(self, indent=0):
+        print(" " * indent, "plane-wave source:")
+        print(" " * indent, "propagation direction:", self.k)
+        print(" " * indent, "center:", self.center)
+        print(" " * indent, "source plane size:", self.size)
+        print(" " * indent, "polarization direction:", self.e_direction)
+        print(" " * indent, "amplitude:", self.amp)
+        self.src_time.display_info(4)
+
+    def mode_function(self, x, y, z):
+        return 1.0
+
+    def _dist_from_center(self, point):
+        return np.linalg.norm(self.center - point)
+
+    def _metric_from_center_along_beam_axis(self, point):
+        return np.dot(self.k, point - self.center)
+
+    def _dist_from_beam_axis(self, x, y, z):
+        return np.linalg.norm(np.cross(self.k, (x, y, z) - self.center))
+
+    def _axis_in_k(self):
+        dot_with_axis = {}
+        for d in (const.PlusX, const.PlusY, const.PlusZ, const.MinusX, const.MinusY, const.MinusZ):
+            dot_with_axis[np.dot(d.vector, self.k)] = d
+        return dot_with_axis[max(dot_with_axis)]
+
+    def _3d_dispersion_relation(self, zeta, v, omega, ds, dt, k):
+        """Taflove & Hagness eq. 5.65 (gmes/source.py:506-526)."""
+        lhs = (sin(0.5 * dt * omega) / v / dt) ** 2
+        rhs = sum((np.sin(0.5 * zeta * np.array(ds) * k) / ds) ** 2)
+        return lhs - rhs
+
+    def _1d_dispersion_relation(self, ds, zeta, v, omega, dt, k):
+        """Taflove & Hagness eq. 5.67 (gmes/source.py:528-547)."""
+        lhs = sin(0.5 * omega * dt) / v / dt
+        if ds == 0:
+            rhs = 0.5 * k * zeta
+        else:
+            rhs = sin(0.5 * k * zeta * ds) / ds
+        return lhs - rhs
+
+    def _get_aux_fdtd(self, space, geom_tree, cmplx):
+        """The 1-D incident grid (gmes/source.py:549-630): its cell size
+        matches the main grid's numerical dispersion along k; a 50-cell
+        CPML and a hard Ex source sit beyond the farthest face."""
+        from scipy.optimize import bisect
+        from .fdtd import TEMzFDTD
+        from .geometry import Cartesian
+        from .material import Cpml
+        from .pygeom import DefaultMedium, Shell
+        default_medium = geom_tree.object_of_point((inf, inf, inf))[0]
+        eps_inf = default_medium.material.eps_inf
+        mu_inf = default_medium.material.mu_inf
+        v = 1 / sqrt(eps_inf * mu_inf)
+        ds = tuple(space.dr)
+        dt = space.dt
+        omega = 2 * pi * self.src_time.freq
+        wave_vector = omega * self.k / v
+        zeta = bisect(self._3d_dispersion_relation, 0, 2, (v, omega, ds, dt, wave_vector))
+        wave_number = omega / v
+        delta_1d = bisect(self._1d_dispersion_relation, 0, 2 * max(ds),
+                          (zeta, v, omega, dt, wave_number))
+        pml_thickness = 50 * delta_1d
+        vertices = []
+        for x in (0.5 * self.size[0], -0.5 * self.size[0]):
+            for y in (0.5 * self.size[1], -0.5 * self.size[1]):
+                for z in (0.5 * self.size[2], -0.5 * self.size[2]):
+                    vertices.append(self.center + (x, y, z))
+        dist = list(map(abs, list(map(self._metric_from_center_along_beam_axis, vertices))))
+        max_dist = max(dist)
+        longitudinal_size = 2 * (max_dist + pml_thickness + 2 * delta_1d)
+        aux_size = (0, 0, longitudinal_size)
+        mat_objs = self.geom_tree.material_of_point((inf, inf, inf))[0]
+        aux_space = Cartesian(size=aux_size, resolution=1 / delta_1d, parallel=False)
+        aux_geom_list = (DefaultMedium(material=mat_objs),
+                         Shell(material=Cpml(), thickness=pml_thickness))
+        src_pnt = (0, 0, -max_dist - delta_1d)
+        aux_src_list = (PointSource(src_time=deepcopy(self.src_time), component=const.Ex,
+                                    center=src_pnt),)
+        return TEMzFDTD(aux_space, list(aux_geom_list), list(aux_src_list), dt=space.dt,
+                        bloch=(0, 0, 0) if cmplx else None, verbose=False,
+                        precision=self._precision, device=self._device, _engine=self._engine)
+
+    def _faces(self, comp):
+        """[(face entry, cosine)] of ``comp`` in the reference's order; faces
+        whose cosine is zero are left out (gmes/source.py:705-1406)."""
+        electric = comp in const.E_COMPONENTS
+        direction = self.h_direction if electric else self.e_direction
+        out = []
+        for entry in _TFSF_FACES.get(comp, ()):
+            cosine = np.dot(direction, entry[3])
+            if cosine != 0:
+                out.append((entry, cosine))
+        return out
+
+    def _envelope_row(self, electric):
+        return None
+
+    def _pw_ops(self, comp, field_shape, space, geom_tree):
+        """TFSF face corrections on ``comp`` (gmes/source.py:632-703 and the
+        face methods), in the reference's face order."""
+        faces = self._faces(comp)
+        if not faces:
+            return []
+        electric = comp in const.E_COMPONENTS
+        aux = self.aux_fdtd
+        aux_comp = const.Hy if electric else const.Ex
+        aux_shape = aux._shape[aux_comp]
+        env = self._envelope_row(electric)
+        d = space.dr
+        out = []
+        for (face, axis, plus, _, idx_comp, lo_off, hi_off, samp_comp, samp_off, sign), cosine \
+                in faces:
+            if not 2 * space.half_size[axis] > space.dr[axis]:
+                continue
+            flip = np.ones(3)
+            flip[axis] = -1
+            if plus:
+                low = self.center - self.half_size * flip
+                high = self.center + self.half_size
+            else:
+                low = self.center - self.half_size
+                high = self.center + self.half_size * flip
+            low_idx = np.array(space.space_to_index(idx_comp, *low)) + lo_off
+            high_idx = np.array([x + 1 for x in space.space_to_index(idx_comp, *high)]) + hi_off
+            for ijk in np.ndindex(tuple(high_idx - low_idx)):
+                idx = tuple(int(v) for v in (np.array(ijk) + low_idx))
+                if not in_range(idx, field_shape, comp):
+                    continue
+                pnt = space.index_to_space(comp, *idx)
+                mat_obj, underneath = geom_tree.material_of_point(pnt)
+                src = mat_obj if underneath is None else underneath
+                eps_mu = float(src.eps_inf if electric else src.mu_inf)
+                amp = cosine * self.amp * self.mode_function(*pnt)
+                samp_pt = space.index_to_space(samp_comp, *(np.array(idx) + samp_off))
+                samp_pnt = (0, 0, self._metric_from_center_along_beam_axis(samp_pt))
+                samp_idx = aux.space.spc_to_exact_idx(aux_comp, *samp_pnt)
+                low_s = np.floor(samp_idx).astype(np.intp)
+                i0 = tuple(int(v) for v in low_s)
+                i1 = (i0[0], i0[1], i0[2] + 1)
+                r1 = samp_idx[2] - floor(samp_idx[2])
+                r0 = 1 - r1
+                for ii in (i0, i1):
+                    if not all(0 <= ii[a] < aux_shape[a] for a in range(3)):
+                        raise ValueError("TFSF sample point outside the auxiliary grid")
+                flat0 = (i0[0] * aux_shape[1] + i0[1]) * aux_shape[2] + i0[2]
+                flat1 = (i1[0] * aux_shape[1] + i1[1]) * aux_shape[2] + i1[2]
+                k_coef = dt_over(space.dt, eps_mu, d[axis]) * amp
+                payload = dict(aux=self._aux_slot, aux_comp=aux_comp.index, i0=flat0, i1=flat1,
+                               k=k_coef, r0=r0, r1=r1, face=face, aux_owner=aux, envelope=env)
+                kind = SRC_AUX_ADD if sign > 0 else SRC_AUX_SUB
+                out.append((idx, "tfsf", kind, payload))
+        return out
+
+
+def dt_over(dt, eps_mu, d):
+    """dt / (eps_inf * d): the face-correction factor of
+    gmes/pw_source.py:212-486."""
+    return dt / (eps_mu * d)
+
+
+class GaussianBeam(TotalFieldScatteredField):
+    """Transparent Gaussian beam launched through one face of the box
+    (gmes/source.py:1525-1769).  The auxiliary grid is first run to a
+    steady state; afterwards the incident samples are scaled by a sin^2
+    turn-on envelope of the source's width, evaluated per half step."""
+
+    def __init__(self, src_time, directivity, center, size, direction, polarization,
+                 waist=inf, amp=1):
+        TotalFieldScatteredField.__init__(self, src_time, center, size, direction,
+                                          polarization, amp)
+        if isinstance(directivity, type) and issubclass(directivity, const.Directional):
+            self.directivity = directivity
+        else:
+            raise TypeError("directivity must be a Directional type.")
+        self.waist = float(waist)
+
+    def init(self, geom_tree, space, cmplx):
+        """gmes/source.py:1583-1603"""
+        self.geom_tree = geom_tree
+        self.src_time.init(cmplx)
+        aux_fdtd = self._get_aux_fdtd(space, geom_tree, cmplx)
+        aux_fdtd.init()
+        raising = aux_fdtd.src_list[0].src_time.width
+        dist = 2 * aux_fdtd.space.half_size[2]
+        from .pygeom import DefaultMedium
+        default_medium = next(i for i in aux_fdtd.geom_list if isinstance(i, DefaultMedium))
+        v_p = 1 / sqrt(default_medium.material.eps_inf * default_medium.material.mu_inf)
+        passby = raising + dist / v_p
+        aux_fdtd.step_until_t(2 * passby)
+        self.aux_fdtd = aux_fdtd
+        self._env_width = raising
+
+    def display_info(self, indent=0):
+        print(" " * indent, "Gaussian beam source:")
+        print(" " * indent, end=" ")
+        print("propagation direction:", self.k, end=" ")
+        print("center:", self.center)
+        print("source plane size:", self.size)
+        print("polarization direction:", self.e_direction)
+        print("beam waist:", self.waist)
+        print("maximum amp.:", self.amp)
+        self.src_time.display_info(indent + 4)
+
+    def mode_function(self, x, y, z):
+        r = self._dist_from_beam_axis(x, y, z)
+        return exp(-((r / self.waist) ** 2))
+
+    def _faces(self, comp):
+        """The one face of ``comp`` whose normal is the directivity axis and
+        which lies on the side the beam enters from; zero cosines are kept
+        (gmes/source.py:1613-1732)."""
+        electric = comp in const.E_COMPONENTS
+        direction = self.h_direction if electric else self.e_direction
+        axis = int(np.argmax(np.abs(self.directivity.vector)))
+        plus_face = self.directivity.vector[axis] < 0
+        for entry in _TFSF_FACES.get(comp, ()):
+            if entry[1] == axis and entry[2] == plus_face:
+                return [(entry, np.dot(direction, entry[3]))]
+        return []
+
+    def _envelope_row(self, electric):
+        """env(t) with t = (number of auxiliary steps taken since the
+        pre-run) * dt: k*dt at the E half of main step k, (k+1)*dt at its H
+        half (gmes/source.py:1734-1769)."""
+        width = self._env_width
+
+        def value(dt, n):
+            m = n - 0.5 if electric else n
+            t = int(m) * dt
+            if t < width:
+                return sin(0.5 * pi * t / width) ** 2
+            return 1
+
+        return WaveRow(value, electric)

@@ -125,12 +125,14 @@ typedef struct {
   int32_t comp;
   int32_t kind;             /* GMES_SRC_*                              */
   int64_t cell;             /* flat C-order index into component comp  */
-  int32_t wave;             /* waveform row (SET/SUB)                  */
+  int32_t wave;             /* waveform row (SET/SUB); for AUX_* -1, or
+                               the row of a real envelope that scales
+                               each aux sample (Gaussian beam turn-on)  */
   int32_t aux;              /* aux slot (AUX_*)                        */
   int32_t aux_comp;         /* component of the aux sim read (AUX_*)   */
   int32_t pad_;
   int64_t i0, i1;           /* flat indices into the aux component     */
-  double k, r0, r1;
+  double k, r0, r1;         /* AUX_*: F +-= k * (r0*aux[i0] + r1*aux[i1]) */
 } gmes_src_op;
 
 /* ---- library ---------------------------------------------------------- */
@@ -172,11 +174,12 @@ int gmes_sim_commit(gmes_sim* sim);
 int gmes_sim_set_sources(gmes_sim* sim, const gmes_src_op* ops, int64_t nops,
                          int32_t nwaves);
 /* Waveform samples for steps [step0, step0+nsteps): values[w][s][re,im].
- * Row w is read by SET/SUB ops with wave == w at absolute step step0+s. */
+ * Row w is read by ops with wave == w at absolute step step0+s. */
 int gmes_sim_set_waveforms(gmes_sim* sim, int64_t step0, int64_t nsteps,
                            const double* values);
 /* Attach a 1-D auxiliary sim (TFSF incident grid) advanced between the E and
- * H half steps of sim; AUX_* ops read its fields.  Returns the slot. */
+ * H half steps of sim; AUX_* ops read its fields.  Returns the slot.  The
+ * aux sim keeps its own step count (it may have been run ahead). */
 int gmes_sim_attach_aux(gmes_sim* sim, gmes_sim* aux, int32_t* slot);
 
 /* ---- probes ----------------------------------------------------------- */

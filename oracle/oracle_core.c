@@ -224,6 +224,9 @@ int gmes_sim_set_sources(gmes_sim *s, const gmes_src_op *ops, int64_t nops, int3
     if ((ops[i].kind == GMES_SRC_SET || ops[i].kind == GMES_SRC_SUB) &&
         (ops[i].wave < 0 || ops[i].wave >= nwaves))
       return ocore_set_error(GMES_EINVAL, "bad waveform row");
+    if ((ops[i].kind == GMES_SRC_AUX_ADD || ops[i].kind == GMES_SRC_AUX_SUB) &&
+        (ops[i].wave < -1 || ops[i].wave >= nwaves))
+      return ocore_set_error(GMES_EINVAL, "bad envelope row");
   }
   free(s->ops);
   s->ops = (gmes_src_op *)malloc(sizeof(gmes_src_op) * (nops ? nops : 1));
@@ -375,6 +378,19 @@ static void apply_sources(gmes_sim *s, int c, int64_t step) {
       const gmes_sim *a = s->aux[op->aux];
       ocv a0 = ocore_get_flat(a, op->aux_comp, op->i0);
       ocv a1 = ocore_get_flat(a, op->aux_comp, op->i1);
+      if (op->wave >= 0) {
+        /* Gaussian-beam turn-on envelope scales each sample first
+           (gmes/source.py:1734-1747) */
+        int64_t row = step - s->wave_step0;
+        if (row < 0 || row >= s->wave_n) {
+          s->err = ocore_set_error(GMES_ERUNTIME, "waveform window does not cover step %lld",
+                                   (long long)step);
+          return;
+        }
+        const double env = s->waves[((size_t)op->wave * s->wave_n + row) * 2];
+        a0.re = env * a0.re; a0.im = env * a0.im;
+        a1.re = env * a1.re; a1.im = env * a1.im;
+      }
       ocv inc = {op->r0 * a0.re + op->r1 * a1.re, op->r0 * a0.im + op->r1 * a1.im};
       if (op->kind == GMES_SRC_AUX_ADD) { F.re = F.re + op->k * inc.re; F.im = F.im + op->k * inc.im; }
       else { F.re = F.re - op->k * inc.re; F.im = F.im - op->k * inc.im; }

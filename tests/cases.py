@@ -196,3 +196,55 @@ def bloch3d(lib, precision="fp64"):
                  _engine=lib)
     sim.init()
     return sim
+
+
+def tfsf2d(lib, precision="fp64", scatterer=True, bloch=None, res=20):
+    """examples/tfsf_with_scatterer.py of the reference (examples/tfsf.py
+    without the cylinder): a CW plane wave along (1,-1,0) through a 3x3
+    TFSF box in a 5x5 TMz cell."""
+    space = g.Cartesian(size=(5, 5, 0), resolution=res)
+    geometry = [g.DefaultMedium(g.Dielectric())]
+    if scatterer:
+        geometry.append(g.Cylinder(g.Dielectric(3), center=(0, 0, 0), radius=1, axis=(0, 0, 1)))
+    geometry.append(g.Shell(material=g.Cpml(), thickness=0.5))
+    sources = [g.TotalFieldScatteredField(src_time=g.Continuous(freq=0.8), center=(0, 0, 0),
+                                          size=(3, 3, 1), direction=(1, -1, 0),
+                                          polarization=(0, 0, 1))]
+    sim = g.TMzFDTD(space, geometry, sources, bloch=bloch, verbose=False, precision=precision,
+                    _engine=lib)
+    sim.init()
+    return sim
+
+
+def tfsf3d(lib, precision="fp64", bloch=None, scatterer=True, direction=(1.0, 0.6, 0.35),
+           polarization=None, center=(0.125, 0, -0.125), size=(1.4375, 1.1875, 1.4375),
+           src_time=None):
+    """3-D TFSF box.  By default the wave vector and polarization are
+    oblique, so every one of the 24 faces carries a correction and edge
+    cells merge two."""
+    space = g.Cartesian(size=(4, 4, 4), resolution=8)
+    geometry = [g.DefaultMedium(g.Dielectric())]
+    if scatterer:
+        geometry.append(g.Sphere(g.Dielectric(4), center=(0.1, -0.05, 0.05), radius=0.4))
+    geometry.append(g.Shell(material=g.Cpml(), thickness=0.5))
+    k = np.array(direction, np.double)
+    e = np.cross(k, (0.2, -1.0, 0.4)) if polarization is None else polarization
+    st = g.Bandpass(freq=0.8, fwidth=1.5, s=3) if src_time is None else src_time
+    sources = [g.TotalFieldScatteredField(src_time=st, center=tuple(center), size=tuple(size),
+                                          direction=tuple(k), polarization=tuple(e))]
+    sim = g.FDTD(space, geometry, sources, bloch=bloch, verbose=False, precision=precision,
+                 _engine=lib)
+    sim.init()
+    return sim
+
+
+def gaussian_beam2d(lib, precision="fp64"):
+    """A Gaussian beam entering a TMz cell through the -x face of its box."""
+    space = g.Cartesian(size=(4, 4, 0), resolution=10)
+    geometry = [g.DefaultMedium(g.Dielectric()), g.Shell(material=g.Cpml(), thickness=0.5)]
+    sources = [g.GaussianBeam(src_time=g.Continuous(freq=0.8), directivity=g.PlusX,
+                              center=(-0.5, 0, 0), size=(0, 2.5, 0), direction=(1, 0, 0),
+                              polarization=(0, 0, 1), waist=0.6)]
+    sim = g.TMzFDTD(space, geometry, sources, verbose=False, precision=precision, _engine=lib)
+    sim.init()
+    return sim

@@ -12,8 +12,8 @@ import pytest
 
 import gmes_amd as g
 from cases import (air2d, bloch2d, bloch3d, const_case, currents, dcp_ade, dcp_plrc, dcp_rc,
-                   dispersive2d, dm2_1d, drude, fields, kat_tmz, lorentz, mode_case, rel_l2,
-                   spheres3d)
+                   dispersive2d, dm2_1d, drude, fields, gaussian_beam2d, kat_tmz, lorentz,
+                   mode_case, rel_l2, spheres3d, tfsf2d, tfsf3d)
 
 pytestmark = pytest.mark.gpu
 
@@ -177,3 +177,39 @@ def test_host_write_before_step(engine_lib, oracle_lib):
             sim.step()
         out.append(fields(sim))
     assert rel_l2(out[0], out[1]) <= FP64_TOL
+
+
+TFSF_CASES = [
+    (tfsf2d, dict(res=10), 150),
+    (tfsf2d, dict(res=10, bloch=(0, 0, 0)), 100),
+    (tfsf3d, {}, 60),
+    (tfsf3d, dict(bloch=(0.3, 0, 0.1)), 40),
+    (gaussian_beam2d, {}, 150),
+]
+TFSF_IDS = ["2d", "2d_complex", "3d_oblique", "3d_bloch", "gaussian_beam"]
+
+
+@pytest.mark.parametrize("builder,kw,steps", TFSF_CASES, ids=TFSF_IDS)
+def test_tfsf(engine_lib, oracle_lib, builder, kw, steps):
+    """TFSF faces and the auxiliary 1-D grid advanced on the device."""
+    gpu = run(builder, engine_lib, steps, **kw)
+    assert max(np.abs(v).max() for v in gpu.values()) > 0.1
+    assert rel_l2(gpu, run(builder, oracle_lib, steps, **kw)) <= FP64_TOL
+
+
+@pytest.mark.parametrize("builder,kw,steps", [TFSF_CASES[0], TFSF_CASES[2]], ids=["2d", "3d"])
+def test_tfsf_fp32(engine_lib, builder, kw, steps):
+    gpu = run(builder, engine_lib, steps, precision="fp32", **kw)
+    assert rel_l2(gpu, run(builder, engine_lib, steps, **kw)) <= 1e-5
+
+
+def test_tfsf_scatterer_full_run(engine_lib, oracle_lib):
+    """examples/tfsf_with_scatterer.py to t = 200 (5714 steps) in FP64."""
+    sim = tfsf2d(engine_lib)
+    sim.step_until_t(200)
+    assert sim.time_step.n == 5714.0
+    ref = tfsf2d(oracle_lib)
+    ref.step_until_t(200)
+    gpu = fields(sim)
+    assert rel_l2(gpu, fields(ref)) <= FP64_TOL
+    assert abs(np.abs(gpu[g.Ez]).max() - 2.0727) / 2.0727 < 2e-4
