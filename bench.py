@@ -90,6 +90,17 @@ def cpu_baseline(n, steps, warm):
                             "(%.1f s)" % (n, steps, warm, dt))
 
 
+def copy_probe():
+    """A plain device-to-device copy of 256 MiB in, 256 MiB out, five times."""
+    import torch
+    src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log("copy probe: 5 x %d bytes" % (src.numel() * 4))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,7 +111,13 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 256 MiB device copy 5 times (scale reference for "
+                         "the FETCH_SIZE/WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print("bench.py: this engine runs one simulation per GPU on one-GPU boxes; "
               "--gpus %d is not supported (use --gpus 1)" % args.gpus, file=sys.stderr)

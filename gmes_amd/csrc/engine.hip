@@ -284,7 +284,7 @@ void enqueue_main(gmes_sim* s, bool electric, hipStream_t st) {
       by = std::max(1, 256 / bx);
     }
     int ic = s->tile_ic;
-    if (ic <= 0) ic = 2;
+    if (ic <= 0) ic = 1;
     time_begin(s, electric ? 0 : 1, st);
     launch_main3d<T>(electric, m, bx, by, ic, st);
     time_end(s, st);

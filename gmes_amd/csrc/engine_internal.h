@@ -25,7 +25,7 @@ template <class R> struct is_cplx<cplx<R>> { static constexpr bool value = true;
 
 // Device-side record: the FP64 host record reduced to what a kernel reads.
 template <class R>
-struct DevRec {
+struct alignas(16) DevRec {
   int32_t cls, n1, n2, prof;
   int64_t coef_off;  // into the device coefficient pool (R)
   R c0;              // dt / eps_mu

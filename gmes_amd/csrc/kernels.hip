@@ -36,6 +36,34 @@ template <class R> __device__ __forceinline__ cplx<R> cmul(cplx<R> a, cplx<R> b)
   return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 template <class T> __device__ __forceinline__ T vzero() { return T{}; }
+
+// Streaming accesses: field words and material ids are touched once per
+// launch, so they are loaded and stored non-temporally (no reuse to keep
+// in L2).  GMES_NT=0 builds the plain form.
+#ifndef GMES_NT
+#define GMES_NT 1
+#endif
+template <class R> __device__ __forceinline__ R ld_nt(const R* p) {
+#if GMES_NT
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+template <class R> __device__ __forceinline__ cplx<R> ld_nt(const cplx<R>* p) {
+  return {ld_nt(&p->re), ld_nt(&p->im)};
+}
+template <class R> __device__ __forceinline__ void st_nt(R* p, R v) {
+#if GMES_NT
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+template <class R> __device__ __forceinline__ void st_nt(cplx<R>* p, cplx<R> v) {
+  st_nt(&p->re, v.re);
+  st_nt(&p->im, v.im);
+}
 template <class R> __device__ __forceinline__ R re_of(R a) { return a; }
 template <class R> __device__ __forceinline__ R re_of(cplx<R> a) { return a.re; }
 template <class R> __device__ __forceinline__ R im_of(R) { return R(0); }
@@ -46,9 +74,6 @@ template <> __device__ __forceinline__ double make_val<double>(double re, double
 template <> __device__ __forceinline__ cplx<float> make_val<cplx<float>>(float re, float im) { return {re, im}; }
 template <> __device__ __forceinline__ cplx<double> make_val<cplx<double>>(double re, double im) { return {re, im}; }
 
-// component geometry: in1, in2, d1 axis, d2 axis, own axis (gmes/fdtd.py:673-809)
-__device__ __constant__ int c_ax1[6] = {1, 2, 0, 1, 2, 0};
-__device__ __constant__ int c_ax2[6] = {2, 0, 1, 2, 0, 1};
 
 template <class T>
 __device__ __forceinline__ T ld(const FieldArgs<T>& a, int c, int i, int j, int k) {
@@ -96,62 +121,37 @@ __device__ __forceinline__ cplx<float> fma_(float a, cplx<float> b, cplx<float> 
   return {__builtin_fmaf(a, b.re, c.re), __builtin_fmaf(a, b.im, c.im)};
 }
 
-// CPML update of one cell in registers (src/pw_cpml.hh:127-132,339-344):
-// psi1, psi2 are updated in place, the new field value is returned.  FP32
-// uses reciprocals (the FP32 profile holds 1/kappa) and FMAs; FP64 keeps
-// the reference's operation order.
+// New value of a Dielectric, CPML or UPML cell of component C from its old
+// value f and curl differences D1, D2.  st is the component's CPML/UPML
+// state array (same layout as the field), flat the cell's element.
+//   FP64 keeps the reference's operation order (division by d1/d2, dt/eps
+//   first, left-to-right sums): src/pw_dielectric.hh:118,292,
+//   src/pw_cpml.hh:127-132,339-344, src/pw_upml.hh:124-128,328-332.
+//   FP32 uses reciprocals the host computed in FP64 (1/d, 1/kappa in the
+//   CPML profile, 1/eps in the record) and explicit FMAs.
 template <int C, class T>
-__device__ __forceinline__ T cpml1(const MainArgs<T>& m, typename real_of<T>::type c0,
-                                   const typename real_of<T>::type* p1, const typename real_of<T>::type* p2,
-                                   T f, T& psi1, T& psi2, T D1, T D2) {
+__device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typename real_of<T>::type>& rec,
+                                          T* __restrict__ st, int flat, int i, int j, int k, T f, T D1, T D2) {
   using R = typename real_of<T>::type;
   constexpr bool E = C < 3;
-  if constexpr (std::is_same<R, float>::value) {
-    const T a = vscl(m.inv_d[Ax<C>::a1], D1), b = vscl(m.inv_d[Ax<C>::a2], D2);
-    psi1 = fma_(p1[1], a, vscl(p1[0], psi1));
-    psi2 = fma_(p2[1], b, vscl(p2[0], psi2));
-    const T t = vsub(vadd(fma_(-p2[2], b, vscl(p1[2], a)), psi1), psi2);
-    return E ? fma_(c0, t, f) : fma_(-c0, t, f);
-  } else {
-    const R d1 = m.d[Ax<C>::a1], d2 = m.d[Ax<C>::a2];
-    psi1 = vadd(vscl(p1[0], psi1), vdiv(vscl(p1[1], D1), d1));
-    psi2 = vadd(vscl(p2[0], psi2), vdiv(vscl(p2[1], D2), d2));
-    const T t = vsub(vadd(vsub(vdiv(vdiv(D1, d1), p1[2]), vdiv(vdiv(D2, d2), p2[2])), psi1), psi2);
-    return E ? vadd(f, vscl(c0, t)) : vsub(f, vscl(c0, t));
-  }
-}
-
-// FP32 form of the Dielectric / CPML / UPML updates: same recursions, with
-// divisions replaced by products with reciprocals the host computed in FP64
-// (profiles hold 1/kappa for CPML; the record holds 1/eps).
-template <int C, class T>
-__device__ __forceinline__ void update_cell_f32(const MainArgs<T>& m, const DevRec<float>& rec, T* F, T f,
-                                                int i, int j, int k, T D1, T D2) {
-  constexpr bool E = C < 3;
   const int cls = rec.cls;
-  const float id1 = m.inv_d[Ax<C>::a1], id2 = m.inv_d[Ax<C>::a2];
-  const T a = vscl(id1, D1), b = vscl(id2, D2);  // D1/d1, D2/d2
-  if (cls == GMES_CLS_DIELECTRIC) {
-    if constexpr (E) f = fma_(rec.c0, vsub(a, b), f);
-    else f = fma_(rec.c0, vsub(b, a), f);
-    *F = f;
-    return;
-  }
-  const int ix[3] = {i, j, k};
-  const int64_t* po = m.prof_off + (rec.prof * 6 + C) * 3;
-  const int flat = (int)(F - m.fa.f[C]);
-  T* st = static_cast<T*>(m.pml[C]);
-  if (cls == GMES_CLS_CPML) {
-    const float* p1 = m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 3;
-    const float* p2 = m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 3;
-    T psi1 = fma_(p1[1], a, vscl(p1[0], st[flat]));
-    T psi2 = fma_(p2[1], b, vscl(p2[0], st[m.dsize[C] + flat]));
-    st[flat] = psi1;
-    st[m.dsize[C] + flat] = psi2;
-    const T t = vsub(vadd(fma_(-p2[2], b, vscl(p1[2], a)), psi1), psi2);
-    if constexpr (E) f = fma_(rec.c0, t, f);
-    else f = fma_(-rec.c0, t, f);
-  } else {  // UPML
+  if constexpr (std::is_same<R, float>::value) {
+    const float id1 = m.inv_d[Ax<C>::a1], id2 = m.inv_d[Ax<C>::a2];
+    const T a = vscl(id1, D1), b = vscl(id2, D2);  // D1/d1, D2/d2
+    if (cls == GMES_CLS_DIELECTRIC) return E ? fma_(rec.c0, vsub(a, b), f) : fma_(rec.c0, vsub(b, a), f);
+    const int ix[3] = {i, j, k};
+    const int64_t* po = m.prof_off + (rec.prof * 6 + C) * 3;
+    if (cls == GMES_CLS_CPML) {
+      const float* p1 = m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 3;
+      const float* p2 = m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 3;
+      const T psi1 = fma_(p1[1], a, vscl(p1[0], st[flat]));
+      const T psi2 = fma_(p2[1], b, vscl(p2[0], st[m.dsize[C] + flat]));
+      st[flat] = psi1;
+      st[m.dsize[C] + flat] = psi2;
+      const T t = vsub(vadd(fma_(-p2[2], b, vscl(p1[2], a)), psi1), psi2);
+      return E ? fma_(rec.c0, t, f) : fma_(-rec.c0, t, f);
+    }
+    // UPML
     const float* p1 = m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 6;
     const float* p2 = m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 6;
     const float* p0 = m.prof + po[Ax<C>::a0] + ix[Ax<C>::a0] * 6;
@@ -159,41 +159,16 @@ __device__ __forceinline__ void update_cell_f32(const MainArgs<T>& m, const DevR
     const T cu = vsub(a, b);
     const T dn = E ? fma_(p1[1], cu, vscl(p1[0], dstore)) : fma_(-p1[1], cu, vscl(p1[0], dstore));
     st[flat] = dn;
-    f = fma_(p2[3] * rec.inv_eps, fma_(-p0[5], dstore, vscl(p0[4], dn)), vscl(p2[2], f));
-  }
-  *F = f;
-}
-
-// Update of one cell of component C given its curl differences D1, D2:
-// Dummy, Const, Dielectric, CPML, UPML.  ADE classes are skipped here.
-template <int C, class T>
-__device__ __forceinline__ void update_cell(const MainArgs<T>& m, const DevRec<typename real_of<T>::type>& rec,
-                                            int flat, int i, int j, int k, T D1, T D2) {
-  using R = typename real_of<T>::type;
-  const int cls = rec.cls;
-  constexpr bool E = C < 3;
-  if (cls == GMES_CLS_DUMMY || cls >= GMES_CLS_DRUDE) return;
-  T* F = m.fa.f[C] + didx(m.fa, C, i, j, k);
-  if (cls == GMES_CLS_CONST) {  // src/pw_const.hh:96,198
-    *F = make_val<T>(rec.val_re, rec.val_im);
-    return;
-  }
-  const R d1 = m.d[Ax<C>::a1], d2 = m.d[Ax<C>::a2];
-  T f = *F;
-  if constexpr (std::is_same<R, float>::value) {
-    // FP32: reciprocals precomputed in FP64 (1/d, 1/kappa, 1/eps) and
-    // explicit FMAs; FP32 runs are checked against FP64 to 1e-5.
-    update_cell_f32<C>(m, rec, F, f, i, j, k, D1, D2);
-    return;
-  }
-  if (cls == GMES_CLS_DIELECTRIC) {  // src/pw_dielectric.hh:118,292
-    if constexpr (E) f = vadd(f, vscl(rec.c0, vsub(vdiv(D1, d1), vdiv(D2, d2))));
-    else f = vadd(f, vscl(rec.c0, vsub(vdiv(D2, d2), vdiv(D1, d1))));
+    return fma_(p2[3] * rec.inv_eps, fma_(-p0[5], dstore, vscl(p0[4], dn)), vscl(p2[2], f));
   } else {
+    const R d1 = m.d[Ax<C>::a1], d2 = m.d[Ax<C>::a2];
+    if (cls == GMES_CLS_DIELECTRIC) {
+      if constexpr (E) return vadd(f, vscl(rec.c0, vsub(vdiv(D1, d1), vdiv(D2, d2))));
+      else return vadd(f, vscl(rec.c0, vsub(vdiv(D2, d2), vdiv(D1, d1))));
+    }
     const int ix[3] = {i, j, k};
     const int64_t* po = m.prof_off + (rec.prof * 6 + C) * 3;
-    T* st = static_cast<T*>(m.pml[C]);
-    if (cls == GMES_CLS_CPML) {  // src/pw_cpml.hh:127-132,339-344
+    if (cls == GMES_CLS_CPML) {
       const R* p1 = m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 3;
       const R* p2 = m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 3;
       T psi1 = st[flat], psi2 = st[m.dsize[C] + flat];
@@ -201,22 +176,48 @@ __device__ __forceinline__ void update_cell(const MainArgs<T>& m, const DevRec<t
       psi2 = vadd(vscl(p2[0], psi2), vdiv(vscl(p2[1], D2), d2));
       st[flat] = psi1;
       st[m.dsize[C] + flat] = psi2;
-      T t = vsub(vadd(vsub(vdiv(vdiv(D1, d1), p1[2]), vdiv(vdiv(D2, d2), p2[2])), psi1), psi2);
-      if constexpr (E) f = vadd(f, vscl(rec.c0, t));
-      else f = vsub(f, vscl(rec.c0, t));
-    } else {  // GMES_CLS_UPML, src/pw_upml.hh:124-128,328-332
-      const R* p1 = m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 6;
-      const R* p2 = m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 6;
-      const R* p0 = m.prof + po[Ax<C>::a0] + ix[Ax<C>::a0] * 6;
-      T dstore = st[flat], dn;
-      T cu = vsub(vdiv(D1, d1), vdiv(D2, d2));
-      if constexpr (E) dn = vadd(vscl(p1[0], dstore), vscl(p1[1], cu));
-      else dn = vsub(vscl(p1[0], dstore), vscl(p1[1], cu));
-      st[flat] = dn;
-      f = vadd(vscl(p2[2], f), vdiv(vscl(p2[3], vsub(vscl(p0[4], dn), vscl(p0[5], dstore))), rec.eps_mu));
+      const T t = vsub(vadd(vsub(vdiv(vdiv(D1, d1), p1[2]), vdiv(vdiv(D2, d2), p2[2])), psi1), psi2);
+      if constexpr (E) return vadd(f, vscl(rec.c0, t));
+      else return vsub(f, vscl(rec.c0, t));
     }
+    // UPML
+    const R* p1 = m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 6;
+    const R* p2 = m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 6;
+    const R* p0 = m.prof + po[Ax<C>::a0] + ix[Ax<C>::a0] * 6;
+    const T dstore = st[flat];
+    const T cu = vsub(vdiv(D1, d1), vdiv(D2, d2));
+    T dn;
+    if constexpr (E) dn = vadd(vscl(p1[0], dstore), vscl(p1[1], cu));
+    else dn = vsub(vscl(p1[0], dstore), vscl(p1[1], cu));
+    st[flat] = dn;
+    return vadd(vscl(p2[2], f), vdiv(vscl(p2[3], vsub(vscl(p0[4], dn), vscl(p0[5], dstore))), rec.eps_mu));
   }
-  *F = f;
+}
+
+// Update of one cell of component C: Dummy (no-op), Const (src/pw_const.hh
+// :96,198), Dielectric, CPML, UPML.  ADE classes are left to their kernels.
+// f is the cell's old value, loaded by the caller.
+template <int C, class T>
+__device__ __forceinline__ void store_cell(const MainArgs<T>& m, const DevRec<typename real_of<T>::type>& rec,
+                                           T* __restrict__ F, T* __restrict__ st, int flat, int i, int j, int k,
+                                           T f, T D1, T D2) {
+  const int cls = rec.cls;
+  if (cls == GMES_CLS_DUMMY || cls >= GMES_CLS_DRUDE) return;
+  if (cls == GMES_CLS_CONST) {
+    st_nt(F + flat, make_val<T>(rec.val_re, rec.val_im));
+    return;
+  }
+  st_nt(F + flat, update_value<C>(m, rec, st, flat, i, j, k, f, D1, D2));
+}
+
+template <int C, class T>
+__device__ __forceinline__ void update_cell(const MainArgs<T>& m, const DevRec<typename real_of<T>::type>& rec,
+                                            int flat, int i, int j, int k, T D1, T D2) {
+  const int cls = rec.cls;
+  if (cls == GMES_CLS_DUMMY || cls >= GMES_CLS_DRUDE) return;
+  T* F = m.fa.f[C];
+  store_cell<C>(m, rec, F, static_cast<T*>(m.pml[C]), flat, i, j, k, cls == GMES_CLS_CONST ? vzero<T>() : F[flat],
+                D1, D2);
 }
 
 template <int C, class T>
@@ -252,76 +253,204 @@ __global__ void __launch_bounds__(256) k_main(MainArgs<T> m) {
 // non-ghost cell of every component of the field type: for E the cells
 // (i,j,k) of Ex, Ey, Ez; for H the cells Hx(i,j+1,k+1), Hy(i+1,j,k+1),
 // Hz(i+1,j+1,k) -- the ghost planes are never updated, so the thread grid
-// is exactly nx*ny*nz.  A block covers a (j,k) tile and marches along i
-// through `ichunk` planes, keeping the plane it will need again (H at i-1
-// for E, E at i+1 for H) in registers; each curl operand is loaded once and
-// shared by the components that use it.  The record table is staged in LDS.
-template <bool E, class T>
-__global__ void __launch_bounds__(256) k_main3d(MainArgs<T> m, int ichunk) {
+// is exactly nx*ny*nz.  A block covers a (j,k) tile of N consecutive i
+// planes; each curl operand is loaded once and shared by the components
+// that use it.  Every field, id and neighbour load of the N planes is
+// issued first, through non-aliasing pointers; the record table is staged
+// into LDS while those loads are in flight, and only then do the dependent
+// record lookups and updates run.
+template <class T, int N>
+struct Planes {
+  T c[7][N];     // curl operands (see load_e / load_h)
+  T o[3][N];     // old values of the three updated components
+  uint16_t r[3][N];
+};
+
+template <class T, int N>
+__device__ __forceinline__ void load_e(Planes<T, N>& q, int i0, int j, int k, const int* B, const int* P,
+                                       const int* W, const T* __restrict__ hx, const T* __restrict__ hy,
+                                       const T* __restrict__ hz, const T* __restrict__ ex, const T* __restrict__ ey,
+                                       const T* __restrict__ ez, const uint16_t* __restrict__ idx,
+                                       const uint16_t* __restrict__ idy, const uint16_t* __restrict__ idz) {
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    const int i = i0 + n;
+    q.c[0][n] = hx[B[3] + i * P[3] + (j + 1) * W[3] + k + 1];  // hx(i, j+1, k+1)
+    q.c[1][n] = hx[B[3] + i * P[3] + j * W[3] + k + 1];        // hx(i, j,   k+1)
+    q.c[2][n] = hx[B[3] + i * P[3] + (j + 1) * W[3] + k];      // hx(i, j+1, k)
+    q.c[3][n] = hy[B[4] + (i + 1) * P[4] + j * W[4] + k + 1];  // hy(i+1, j, k+1)
+    q.c[4][n] = hy[B[4] + (i + 1) * P[4] + j * W[4] + k];      // hy(i+1, j, k)
+    q.c[5][n] = hz[B[5] + (i + 1) * P[5] + (j + 1) * W[5] + k];  // hz(i+1, j+1, k)
+    q.c[6][n] = hz[B[5] + (i + 1) * P[5] + j * W[5] + k];        // hz(i+1, j,   k)
+    const int fx = B[0] + i * P[0] + j * W[0] + k, fy = B[1] + i * P[1] + j * W[1] + k;
+    const int fz = B[2] + i * P[2] + j * W[2] + k;
+    q.o[0][n] = ld_nt(ex + fx);
+    q.o[1][n] = ld_nt(ey + fy);
+    q.o[2][n] = ld_nt(ez + fz);
+    q.r[0][n] = ld_nt(idx + fx);
+    q.r[1][n] = ld_nt(idy + fy);
+    q.r[2][n] = ld_nt(idz + fz);
+  }
+}
+
+// prev: H'z(i0-1,j,k) = Hz[i0, j+1, k] and H'y(i0-1,j,k) = Hy[i0, j, k+1],
+// the i-1 operands of the first plane; advanced to the last plane's.
+template <class T, int N>
+__device__ __forceinline__ void update_e(const MainArgs<T>& m, const DevRec<typename real_of<T>::type>* recs,
+                                         const Planes<T, N>& q, int i0, int j, int k, const int* B, const int* P,
+                                         const int* W, T* __restrict__ ex, T* __restrict__ ey, T* __restrict__ ez,
+                                         T* __restrict__ sx, T* __restrict__ sy, T* __restrict__ sz, T& hz_prev,
+                                         T& hy_prev) {
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    const int i = i0 + n;
+    const int fx = B[0] + i * P[0] + j * W[0] + k, fy = B[1] + i * P[1] + j * W[1] + k;
+    const int fz = B[2] + i * P[2] + j * W[2] + k;
+    store_cell<0>(m, recs[q.r[0][n]], ex, sx, fx, i, j, k, q.o[0][n], vsub(q.c[5][n], q.c[6][n]),
+                  vsub(q.c[3][n], q.c[4][n]));
+    store_cell<1>(m, recs[q.r[1][n]], ey, sy, fy, i, j, k, q.o[1][n], vsub(q.c[0][n], q.c[2][n]),
+                  vsub(q.c[5][n], hz_prev));
+    store_cell<2>(m, recs[q.r[2][n]], ez, sz, fz, i, j, k, q.o[2][n], vsub(q.c[3][n], hy_prev),
+                  vsub(q.c[0][n], q.c[1][n]));
+    hz_prev = q.c[5][n];
+    hy_prev = q.c[3][n];
+  }
+}
+
+template <class T, int N>
+__device__ __forceinline__ void load_h(Planes<T, N>& q, int i0, int j, int k, const int* B, const int* P,
+                                       const int* W, const T* __restrict__ ex, const T* __restrict__ ey,
+                                       const T* __restrict__ ez, const T* __restrict__ hx, const T* __restrict__ hy,
+                                       const T* __restrict__ hz, const uint16_t* __restrict__ idx,
+                                       const uint16_t* __restrict__ idy, const uint16_t* __restrict__ idz) {
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    const int i = i0 + n;
+    q.c[0][n] = ex[B[0] + i * P[0] + j * W[0] + k];            // ex(i, j, k)
+    q.c[1][n] = ex[B[0] + i * P[0] + j * W[0] + k + 1];        // ex(i, j, k+1)
+    q.c[2][n] = ex[B[0] + i * P[0] + (j + 1) * W[0] + k];      // ex(i, j+1, k)
+    q.c[3][n] = ey[B[1] + i * P[1] + j * W[1] + k + 1];        // ey(i, j, k+1)
+    q.c[4][n] = ey[B[1] + (i + 1) * P[1] + j * W[1] + k];      // ey(i+1, j, k)
+    q.c[5][n] = ez[B[2] + i * P[2] + (j + 1) * W[2] + k];      // ez(i, j+1, k)
+    q.c[6][n] = ez[B[2] + (i + 1) * P[2] + j * W[2] + k];      // ez(i+1, j, k)
+    const int fx = B[3] + i * P[3] + (j + 1) * W[3] + k + 1;    // Hx(i, j+1, k+1)
+    const int fy = B[4] + (i + 1) * P[4] + j * W[4] + k + 1;    // Hy(i+1, j, k+1)
+    const int fz = B[5] + (i + 1) * P[5] + (j + 1) * W[5] + k;  // Hz(i+1, j+1, k)
+    q.o[0][n] = ld_nt(hx + fx);
+    q.o[1][n] = ld_nt(hy + fy);
+    q.o[2][n] = ld_nt(hz + fz);
+    q.r[0][n] = ld_nt(idx + fx);
+    q.r[1][n] = ld_nt(idy + fy);
+    q.r[2][n] = ld_nt(idz + fz);
+  }
+}
+
+// cur: Ey(i0,j,k), Ez(i0,j,k), advanced to the last plane's i+1 values.
+template <class T, int N>
+__device__ __forceinline__ void update_h(const MainArgs<T>& m, const DevRec<typename real_of<T>::type>* recs,
+                                         const Planes<T, N>& q, int i0, int j, int k, const int* B, const int* P,
+                                         const int* W, T* __restrict__ hx, T* __restrict__ hy, T* __restrict__ hz,
+                                         T* __restrict__ sx, T* __restrict__ sy, T* __restrict__ sz, T& ey_cur,
+                                         T& ez_cur) {
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    const int i = i0 + n;
+    const int fx = B[3] + i * P[3] + (j + 1) * W[3] + k + 1;
+    const int fy = B[4] + (i + 1) * P[4] + j * W[4] + k + 1;
+    const int fz = B[5] + (i + 1) * P[5] + (j + 1) * W[5] + k;
+    store_cell<3>(m, recs[q.r[0][n]], hx, sx, fx, i, j + 1, k + 1, q.o[0][n], vsub(q.c[5][n], ez_cur),
+                  vsub(q.c[3][n], ey_cur));
+    store_cell<4>(m, recs[q.r[1][n]], hy, sy, fy, i + 1, j, k + 1, q.o[1][n], vsub(q.c[1][n], q.c[0][n]),
+                  vsub(q.c[6][n], ez_cur));
+    store_cell<5>(m, recs[q.r[2][n]], hz, sz, fz, i + 1, j + 1, k, q.o[2][n], vsub(q.c[4][n], ey_cur),
+                  vsub(q.c[2][n], q.c[0][n]));
+    ey_cur = q.c[4][n];
+    ez_cur = q.c[6][n];
+  }
+}
+
+// Record table staging: the table is copied as 16-byte chunks, one per
+// thread, loaded into registers first and written to LDS later.
+template <class R>
+struct RecStage {
+  static constexpr int kChunks = sizeof(DevRec<R>) / 16;
+  static_assert(sizeof(DevRec<R>) % 16 == 0, "record size must be a multiple of 16 bytes");
+  int4 v;
+  int t;
+  bool use_lds;
+  __device__ __forceinline__ void load(const DevRec<R>* g, int nrec) {
+    t = threadIdx.y * blockDim.x + threadIdx.x;
+    use_lds = nrec * kChunks <= (int)(blockDim.x * blockDim.y) && nrec <= kRecLds;
+    if (use_lds && t < nrec * kChunks) v = reinterpret_cast<const int4*>(g)[t];
+  }
+  __device__ __forceinline__ const DevRec<R>* commit(DevRec<R>* lds, const DevRec<R>* g, int nrec) {
+    if (!use_lds) return g;
+    if (t < nrec * kChunks) reinterpret_cast<int4*>(lds)[t] = v;
+    __syncthreads();
+    return lds;
+  }
+};
+
+template <bool E, class T, int N>
+__global__ void __launch_bounds__(256) k_main3d(MainArgs<T> m) {
   using R = typename real_of<T>::type;
   __shared__ DevRec<R> srec[kRecLds];
-  const DevRec<R>* recs = m.recs;
-  if (m.nrec <= kRecLds) {
-    for (int t = threadIdx.y * blockDim.x + threadIdx.x; t < m.nrec; t += blockDim.x * blockDim.y)
-      srec[t] = m.recs[t];
-    __syncthreads();
-    recs = srec;
-  }
   const int nx = m.n[0], ny = m.n[1], nz = m.n[2];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  if (k >= nz || j >= ny) return;
-  const int i0 = blockIdx.z * ichunk;
-  const int i1 = min(i0 + ichunk, nx);
-  T* const* f = m.fa.f;
+  const bool act = k < nz && j < ny;  // every thread reaches the barrier
+  const int i0 = blockIdx.z * N;
   const FieldArgs<T>& a = m.fa;
   // per component: element of (i,j,k) = B + i*P + j*W + k (pitched layout)
   int B[6], P[6], W[6];
   for (int c = 0; c < 6; ++c) { B[c] = kPad - a.koff[c]; W[c] = a.p[c]; P[c] = a.s[c][1] * a.p[c]; }
-  if constexpr (E) {
-    const T* hx = f[3];
-    const T* hy = f[4];
-    const T* hz = f[5];
-    T hz_prev = hz[B[5] + i0 * P[5] + (j + 1) * W[5] + k];  // H'z(i0-1,j,k) = Hz[i0, j+1, k]
-    T hy_prev = hy[B[4] + i0 * P[4] + j * W[4] + k + 1];    // H'y(i0-1,j,k) = Hy[i0, j, k+1]
-    for (int i = i0; i < i1; ++i) {
-      const T hxc = hx[B[3] + i * P[3] + (j + 1) * W[3] + k + 1];
-      const T hx_jm = hx[B[3] + i * P[3] + j * W[3] + k + 1];
-      const T hx_km = hx[B[3] + i * P[3] + (j + 1) * W[3] + k];
-      const T hyc = hy[B[4] + (i + 1) * P[4] + j * W[4] + k + 1];
-      const T hy_km = hy[B[4] + (i + 1) * P[4] + j * W[4] + k];
-      const T hzc = hz[B[5] + (i + 1) * P[5] + (j + 1) * W[5] + k];
-      const T hz_jm = hz[B[5] + (i + 1) * P[5] + j * W[5] + k];
-      const int fx = B[0] + i * P[0] + j * W[0] + k, fy = B[1] + i * P[1] + j * W[1] + k;
-      const int fz = B[2] + i * P[2] + j * W[2] + k;
-      update_cell<0>(m, recs[m.ids[0][fx]], fx, i, j, k, vsub(hzc, hz_jm), vsub(hyc, hy_km));
-      update_cell<1>(m, recs[m.ids[1][fy]], fy, i, j, k, vsub(hxc, hx_km), vsub(hzc, hz_prev));
-      update_cell<2>(m, recs[m.ids[2][fz]], fz, i, j, k, vsub(hyc, hy_prev), vsub(hxc, hx_jm));
-      hz_prev = hzc;
-      hy_prev = hyc;
+  T* const* f = a.f;
+  T* sx = static_cast<T*>(m.pml[E ? 0 : 3]);
+  T* sy = static_cast<T*>(m.pml[E ? 1 : 4]);
+  T* sz = static_cast<T*>(m.pml[E ? 2 : 5]);
+  const uint16_t* ida = m.ids[E ? 0 : 3];
+  const uint16_t* idb = m.ids[E ? 1 : 4];
+  const uint16_t* idc = m.ids[E ? 2 : 5];
+  RecStage<R> stage;
+  stage.load(m.recs, m.nrec);
+  if (i0 + N <= nx) {
+    Planes<T, N> q;
+    T u0 = T{}, u1 = T{};
+    if (act) {
+      if constexpr (E) {
+        load_e<T, N>(q, i0, j, k, B, P, W, f[3], f[4], f[5], f[0], f[1], f[2], ida, idb, idc);
+        u0 = f[5][B[5] + i0 * P[5] + (j + 1) * W[5] + k];
+        u1 = f[4][B[4] + i0 * P[4] + j * W[4] + k + 1];
+      } else {
+        load_h<T, N>(q, i0, j, k, B, P, W, f[0], f[1], f[2], f[3], f[4], f[5], ida, idb, idc);
+        u0 = f[1][B[1] + i0 * P[1] + j * W[1] + k];
+        u1 = f[2][B[2] + i0 * P[2] + j * W[2] + k];
+      }
     }
+    const DevRec<R>* recs = stage.commit(srec, m.recs, m.nrec);
+    if (!act) return;
+    if constexpr (E) update_e<T, N>(m, recs, q, i0, j, k, B, P, W, f[0], f[1], f[2], sx, sy, sz, u0, u1);
+    else update_h<T, N>(m, recs, q, i0, j, k, B, P, W, f[3], f[4], f[5], sx, sy, sz, u0, u1);
   } else {
-    const T* ex = f[0];
-    const T* ey = f[1];
-    const T* ez = f[2];
-    T ey_cur = ey[B[1] + i0 * P[1] + j * W[1] + k];
-    T ez_cur = ez[B[2] + i0 * P[2] + j * W[2] + k];
-    for (int i = i0; i < i1; ++i) {
-      const T exc = ex[B[0] + i * P[0] + j * W[0] + k];
-      const T ex_kp = ex[B[0] + i * P[0] + j * W[0] + k + 1];
-      const T ex_jp = ex[B[0] + i * P[0] + (j + 1) * W[0] + k];
-      const T ey_kp = ey[B[1] + i * P[1] + j * W[1] + k + 1];
-      const T ey_ip = ey[B[1] + (i + 1) * P[1] + j * W[1] + k];
-      const T ez_jp = ez[B[2] + i * P[2] + (j + 1) * W[2] + k];
-      const T ez_ip = ez[B[2] + (i + 1) * P[2] + j * W[2] + k];
-      const int fx = B[3] + i * P[3] + (j + 1) * W[3] + k + 1;    // Hx(i, j+1, k+1)
-      const int fy = B[4] + (i + 1) * P[4] + j * W[4] + k + 1;    // Hy(i+1, j, k+1)
-      const int fz = B[5] + (i + 1) * P[5] + (j + 1) * W[5] + k;  // Hz(i+1, j+1, k)
-      update_cell<3>(m, recs[m.ids[3][fx]], fx, i, j + 1, k + 1, vsub(ez_jp, ez_cur), vsub(ey_kp, ey_cur));
-      update_cell<4>(m, recs[m.ids[4][fy]], fy, i + 1, j, k + 1, vsub(ex_kp, exc), vsub(ez_ip, ez_cur));
-      update_cell<5>(m, recs[m.ids[5][fz]], fz, i + 1, j + 1, k, vsub(ey_ip, ey_cur), vsub(ex_jp, exc));
-      ey_cur = ey_ip;
-      ez_cur = ez_ip;
+    const DevRec<R>* recs = stage.commit(srec, m.recs, m.nrec);
+    if (!act) return;
+    T u0, u1;
+    if constexpr (E) {
+      u0 = f[5][B[5] + i0 * P[5] + (j + 1) * W[5] + k];
+      u1 = f[4][B[4] + i0 * P[4] + j * W[4] + k + 1];
+    } else {
+      u0 = f[1][B[1] + i0 * P[1] + j * W[1] + k];
+      u1 = f[2][B[2] + i0 * P[2] + j * W[2] + k];
+    }
+    for (int i = i0; i < nx; ++i) {
+      Planes<T, 1> q;
+      if constexpr (E) {
+        load_e<T, 1>(q, i, j, k, B, P, W, f[3], f[4], f[5], f[0], f[1], f[2], ida, idb, idc);
+        update_e<T, 1>(m, recs, q, i, j, k, B, P, W, f[0], f[1], f[2], sx, sy, sz, u0, u1);
+      } else {
+        load_h<T, 1>(q, i, j, k, B, P, W, f[0], f[1], f[2], f[3], f[4], f[5], ida, idb, idc);
+        update_h<T, 1>(m, recs, q, i, j, k, B, P, W, f[3], f[4], f[5], sx, sy, sz, u0, u1);
+      }
     }
   }
 }
@@ -666,10 +795,17 @@ void launch_main(bool electric, const MainArgs<T>& m, int ext[3], int bx, int by
 
 template <class T>
 void launch_main3d(bool electric, const MainArgs<T>& m, int bx, int by, int ichunk, hipStream_t st) {
-  dim3 block(bx, by, 1);
-  dim3 grid((m.n[2] + bx - 1) / bx, (m.n[1] + by - 1) / by, (m.n[0] + ichunk - 1) / ichunk);
-  if (electric) hipLaunchKernelGGL((k_main3d<true, T>), grid, block, 0, st, m, ichunk);
-  else hipLaunchKernelGGL((k_main3d<false, T>), grid, block, 0, st, m, ichunk);
+  dim3 block(bx, by);
+  const int n = ichunk <= 1 ? 1 : ichunk <= 2 ? 2 : ichunk <= 4 ? 4 : 8;
+  dim3 grid((m.n[2] + bx - 1) / bx, (m.n[1] + by - 1) / by, (m.n[0] + n - 1) / n);
+#define GMES_L3(NN)                                                                    \
+  if (n == NN) {                                                                       \
+    if (electric) hipLaunchKernelGGL((k_main3d<true, T, NN>), grid, block, 0, st, m);  \
+    else hipLaunchKernelGGL((k_main3d<false, T, NN>), grid, block, 0, st, m);          \
+    return;                                                                            \
+  }
+  GMES_L3(1) GMES_L3(2) GMES_L3(4) GMES_L3(8)
+#undef GMES_L3
 }
 
 template <class T>
