@@ -446,6 +446,7 @@ static int one_step(gmes_sim *s) {
 
 int gmes_sim_step(gmes_sim *s, int64_t nsteps) {
   if (!s->committed) return ocore_set_error(GMES_EINVAL, "sim not committed");
+  if (nsteps < 0) return ocore_set_error(GMES_EINVAL, "negative step count");
   if (s->err) return s->err;
   for (int64_t t = 0; t < nsteps; ++t) {
     int rc = one_step(s);
