@@ -88,6 +88,7 @@ _SIGNATURES = {
     "gmes_sim_get_step": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "gmes_sim_set_graph": (C.c_int, [_P, C.c_int32]),
     "gmes_sim_set_tile": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "gmes_sim_autotune": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "gmes_sim_profile": (C.c_int, [_P, C.c_int32]),
     "gmes_sim_read_profile": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "gmes_batch_create": (C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(_P)]),
@@ -221,6 +222,11 @@ class Library(object):
 
     def set_tile(self, h, bx, by):
         self.call("gmes_sim_set_tile", h, bx, by)
+
+    def autotune(self, h):
+        bx, by = C.c_int32(), C.c_int32()
+        self.call("gmes_sim_autotune", h, C.byref(bx), C.byref(by))
+        return bx.value, by.value
 
     def profile(self, h, enable):
         self.call("gmes_sim_profile", h, int(enable))

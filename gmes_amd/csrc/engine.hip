@@ -12,7 +12,10 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <numeric>
+#include <string>
 #include <vector>
 
 #include "engine_kernels.h"
@@ -225,13 +228,14 @@ void time_end(gmes_sim* s, hipStream_t st) {
   (void)hipEventRecord(s->timing.pending.back().second.second, st);
 }
 
+// Ghost-copy arguments of one field type; total = ghost points.
 template <class T>
-void enqueue_ghost(gmes_sim* s, bool electric, int tick, hipStream_t st) {
+GhostArgs<T> ghost_args(const gmes_sim* s, bool electric, int tick, int64_t& total) {
   using R = typename real_of<T>::type;
   GhostArgs<T> g{};
   g.fa = field_args<T>(s);
   g.ncomp = 0;
-  int64_t total = 0;
+  total = 0;
   for (int c = electric ? 0 : 3; c < (electric ? 3 : 6); ++c) {
     if (!active(s, c)) continue;
     const int t = g.ncomp++;
@@ -245,24 +249,33 @@ void enqueue_ghost(gmes_sim* s, bool electric, int tick, hipStream_t st) {
   }
   g.tick = tick;
   g.ctl = reinterpret_cast<Ctl*>(s->d_clock);
+  return g;
+}
+
+template <class T>
+void enqueue_ghost(gmes_sim* s, bool electric, int tick, hipStream_t st) {
+  int64_t total = 0;
+  GhostArgs<T> g = ghost_args<T>(s, electric, tick, total);
   time_begin(s, 4, st);
   launch_ghost<T>(g, total, st);
   time_end(s, st);
 }
 
+// Main-kernel arguments of one field type; ext = the largest storage extent
+// of its components.  False when the field type has no active component.
 template <class T>
-void enqueue_main(gmes_sim* s, bool electric, hipStream_t st) {
+bool main_args(const gmes_sim* s, bool electric, MainArgs<T>& m, int ext[3]) {
   using R = typename real_of<T>::type;
-  MainArgs<T> m{};
+  m = MainArgs<T>{};
   m.fa = field_args<T>(s);
-  int ext[3] = {0, 0, 0};
+  ext[0] = ext[1] = ext[2] = 0;
   bool any = false;
   for (int c = electric ? 0 : 3; c < (electric ? 3 : 6); ++c) {
     if (!active(s, c)) continue;
     any = true;
     for (int d = 0; d < 3; ++d) ext[d] = std::max<int>(ext[d], (int)s->shape[c][d]);
   }
-  if (!any) return;
+  if (!any) return false;
   for (int c = 0; c < 6; ++c) {
     m.ids[c] = s->ids[c];
     m.pml[c] = s->pml_state[c];
@@ -275,27 +288,47 @@ void enqueue_main(gmes_sim* s, bool electric, hipStream_t st) {
   m.inv_d[0] = (R)(1.0 / s->g.dx); m.inv_d[1] = (R)(1.0 / s->g.dy); m.inv_d[2] = (R)(1.0 / s->g.dz);
   m.nrec = (int)s->recs.size();
   m.n[0] = s->g.nx; m.n[1] = s->g.ny; m.n[2] = s->g.nz;
-  int bx = s->tile_bx, by = s->tile_by;
-  if (s->g.active == 0x3Fu) {
-    // 3-D: (j,k) tiles marching along i; enough blocks to fill 256 CUs.
-    // The vector kernel needs every row padded and nz a multiple of 4.
-    if (bx <= 0 || by <= 0) {
-      bx = std::min(128, next_pow2(s->g.nz));
-      by = std::max(1, 256 / bx);
-    }
-    int ic = s->tile_ic;
-    if (ic <= 0) ic = 1;
-    time_begin(s, electric ? 0 : 1, st);
-    launch_main3d<T>(electric, m, bx, by, ic, st);
-    time_end(s, st);
-    return;
-  }
-  if (bx <= 0 || by <= 0) {
+  return true;
+}
+
+bool is3d(const gmes_sim* s) { return s->g.active == 0x3Fu; }
+
+// Tile (bx along k, by along j) of the main kernels: the sim's own setting
+// (gmes_sim_set_tile, gmes_sim_autotune) or the default.  3-D: (j,k) tiles
+// of one i plane, 128 x 2 for long rows.  Other modes: a k x j tile of one
+// i plane.
+void tile_of(const gmes_sim* s, const int ext[3], int& bx, int& by) {
+  bx = s->tile_bx;
+  by = s->tile_by;
+  if (bx > 0 && by > 0) return;
+  if (is3d(s)) {
+    bx = std::min(128, next_pow2(s->g.nz));
+    by = std::max(1, 256 / bx);
+  } else {
     bx = std::min(64, next_pow2(ext[2]));
     by = std::max(1, std::min(256 / bx, next_pow2(ext[1])));
   }
+}
+
+template <class T>
+void launch_main_tile(const gmes_sim* s, bool electric, const MainArgs<T>& m, int ext[3], int bx, int by,
+                      hipStream_t st) {
+  if (is3d(s)) {
+    launch_main3d<T>(electric, m, bx, by, s->tile_ic > 0 ? s->tile_ic : 1, st);
+  } else {
+    launch_main<T>(electric, m, ext, bx, by, st);
+  }
+}
+
+template <class T>
+void enqueue_main(gmes_sim* s, bool electric, hipStream_t st) {
+  MainArgs<T> m;
+  int ext[3];
+  if (!main_args<T>(s, electric, m, ext)) return;
+  int bx, by;
+  tile_of(s, ext, bx, by);
   time_begin(s, electric ? 0 : 1, st);
-  launch_main<T>(electric, m, ext, bx, by, st);
+  launch_main_tile<T>(s, electric, m, ext, bx, by, st);
   time_end(s, st);
 }
 
@@ -321,18 +354,23 @@ void enqueue_ade(gmes_sim* s, hipStream_t st) {
 }
 
 template <class T>
-void enqueue_src(gmes_sim* s, bool electric, hipStream_t st) {
+SrcArgs<T> src_args(const gmes_sim* s, bool electric) {
   using R = typename real_of<T>::type;
-  const int ng = electric ? s->ngroups_e : s->ngroups_h;
-  if (ng == 0) return;
   SrcArgs<T> a{};
   a.fa = field_args<T>(s);
   a.groups = s->d_groups + (electric ? 0 : s->ngroups_e);
-  a.ngroups = ng;
+  a.ngroups = electric ? s->ngroups_e : s->ngroups_h;
   a.ops = static_cast<const DevSrcOp<R>*>(s->d_ops);
   a.waves = s->d_waves;
   a.aux = static_cast<void* const*>(s->d_aux_ptrs);
   a.ctl = reinterpret_cast<Ctl*>(s->d_clock);
+  return a;
+}
+
+template <class T>
+void enqueue_src(gmes_sim* s, bool electric, hipStream_t st) {
+  const SrcArgs<T> a = src_args<T>(s, electric);
+  if (a.ngroups == 0) return;
   time_begin(s, 3, st);
   launch_src<T>(a, st);
   time_end(s, st);
@@ -353,6 +391,18 @@ void enqueue_probe(gmes_sim* s, bool electric, hipStream_t st) {
   launch_probe<T>(a, st);
 }
 
+void enqueue_step(gmes_sim* s, hipStream_t st);
+
+// One whole step of each auxiliary (TFSF) grid of s, between the E and H
+// half steps of s.
+void enqueue_aux_steps(gmes_sim* s, hipStream_t st) {
+  for (int i = 0; i < s->naux; ++i) {
+    time_begin(s, 5, st);
+    enqueue_step(s->aux[i], st);
+    time_end(s, st);
+  }
+}
+
 template <class T>
 void enqueue_step_t(gmes_sim* s, hipStream_t st) {
   enqueue_ghost<T>(s, false, 1, st);
@@ -360,17 +410,7 @@ void enqueue_step_t(gmes_sim* s, hipStream_t st) {
   enqueue_ade<T>(s, st);
   enqueue_src<T>(s, true, st);
   enqueue_probe<T>(s, true, st);
-  for (int i = 0; i < s->naux; ++i) {
-    gmes_sim* a = s->aux[i];
-    time_begin(s, 5, st);
-    switch (a->g.dtype) {
-      case GMES_F32: enqueue_step_t<float>(a, st); break;
-      case GMES_F64: enqueue_step_t<double>(a, st); break;
-      case GMES_C64: enqueue_step_t<cplx<float>>(a, st); break;
-      default: enqueue_step_t<cplx<double>>(a, st); break;
-    }
-    time_end(s, st);
-  }
+  enqueue_aux_steps(s, st);
   enqueue_ghost<T>(s, true, 0, st);
   enqueue_main<T>(s, false, st);
   enqueue_src<T>(s, false, st);
@@ -392,11 +432,12 @@ void advance_host_clock(gmes_sim* s, int64_t n) {
   for (int i = 0; i < s->naux; ++i) advance_host_clock(s->aux[i], n);
 }
 
-// Waits for all work that may touch s: its own stream and, for an aux sim,
-// the stream of the sim that advances it.
+// Waits for all work that may touch s: its own stream, for an aux sim the
+// stream of the sim that advances it, and the stream of s's batch.
 hipError_t sync_exec(gmes_sim* s) {
   hipError_t e = hipStreamSynchronize(s->stream);
-  if (e == hipSuccess && s->parent) e = hipStreamSynchronize(s->parent->stream);
+  if (e == hipSuccess && s->parent) e = sync_exec(s->parent);
+  if (e == hipSuccess && s->batch_stream) e = hipStreamSynchronize(s->batch_stream);
   return e;
 }
 
@@ -441,7 +482,11 @@ int ensure_probe_capacity(gmes_sim* s, int64_t n) {
   s->d_probe_buf = nb;
   s->probe_cap = cap;
   s->graph_dirty = true;
-  if (s->parent) s->parent->graph_dirty = true;
+  ++s->gen;
+  if (s->parent) {
+    s->parent->graph_dirty = true;
+    ++s->parent->gen;
+  }
   return GMES_OK;
 }
 
@@ -451,6 +496,7 @@ void destroy_graph(gmes_sim* s) {
     s->graph_exec = nullptr;
   }
   s->graph_dirty = true;
+  ++s->gen;
   // the parent's graph holds this aux sim's kernels and their arguments
   if (s->parent) destroy_graph(s->parent);
 }
@@ -487,6 +533,8 @@ bool any_graph_blocker(gmes_sim* s) { return s->timing.on; }
 int do_step(gmes_sim* s, int64_t n) {
   if (!s->committed) return set_error(GMES_EINVAL, "sim is not committed");
   if (n < 0) return set_error(GMES_EINVAL, "negative step count");
+  if (s->parent) return set_error(GMES_EINVAL, "an auxiliary grid is advanced by its parent");
+  if (s->batch_stream) HIP_TRY(hipStreamSynchronize(s->batch_stream));
   int rc = check_windows(s, n);
   if (rc) return rc;
   if ((rc = ensure_probe_capacity(s, n))) return rc;
@@ -502,6 +550,96 @@ int do_step(gmes_sim* s, int64_t n) {
   advance_host_clock(s, n);
   s->probe_n = s->step - s->probe_base;
   if (s->timing.on && s->timing.pending.size() > 4096) return collect_timing(s);
+  return GMES_OK;
+}
+
+// ---- tile autotuner
+// Times the E main kernel of a sim under each candidate tile and keeps the
+// fastest, per size class (device, storage type, mode, log2 of the
+// extents) for the life of the process.  The fields and CPML state the
+// trial launches touch are saved first and restored afterwards, so tuning
+// leaves the simulation unchanged.
+std::mutex g_tune_mu;
+std::map<std::string, std::pair<int, int>> g_tune_cache;
+
+int ceil_log2(int v) {
+  int l = 0;
+  while ((1 << l) < v) ++l;
+  return l;
+}
+
+template <class T>
+int autotune_t(gmes_sim* s, int& best_bx, int& best_by) {
+  MainArgs<T> m;
+  int ext[3];
+  best_bx = best_by = 0;
+  if (!main_args<T>(s, true, m, ext)) return GMES_OK;
+  char key[96];
+  snprintf(key, sizeof key, "%d/%d/%x/%d/%d/%d", s->device, s->g.dtype, s->g.active, ceil_log2(ext[0]),
+           ceil_log2(ext[1]), ceil_log2(ext[2]));
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      best_bx = it->second.first;
+      best_by = it->second.second;
+      return GMES_OK;
+    }
+  }
+  std::vector<int> cand;
+  const int row = next_pow2(ext[2]);
+  for (int bx = 1; bx <= 256; bx *= 2) {
+    if (bx > std::max(row, 1) && bx > 1) continue;          // wider than a row: idle lanes
+    if (row >= 32 && bx < 32) continue;                      // narrower than a 128-byte line
+    cand.push_back(bx);
+  }
+  if (cand.empty()) cand.push_back(std::min(256, row));
+  HIP_TRY(sync_exec(s));
+  // save what the E kernel writes: E fields and E-side PML state
+  std::vector<std::pair<void*, std::pair<void*, size_t>>> saved;
+  for (int c = 0; c < 3; ++c) {
+    if (!active(s, c)) continue;
+    const size_t fb = (size_t)s->dsize[c] * s->esize;
+    void* tmp = nullptr;
+    HIP_TRY(hipMalloc(&tmp, fb));
+    HIP_TRY(hipMemcpy(tmp, s->f[c], fb, hipMemcpyDeviceToDevice));
+    saved.push_back({tmp, {s->f[c], fb}});
+    if (s->pml_state[c]) {
+      const int pw = s->has_cls[c][GMES_CLS_CPML] ? 2 : 1;
+      const size_t pb = (size_t)pw * s->dsize[c] * s->esize;
+      HIP_TRY(hipMalloc(&tmp, pb));
+      HIP_TRY(hipMemcpy(tmp, s->pml_state[c], pb, hipMemcpyDeviceToDevice));
+      saved.push_back({tmp, {s->pml_state[c], pb}});
+    }
+  }
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  float best = 1e30f;
+  const int reps = 5;
+  for (int bx : cand) {
+    const int by = std::max(1, 256 / bx);
+    launch_main_tile<T>(s, true, m, ext, bx, by, s->stream);
+    HIP_TRY(hipEventRecord(e0, s->stream));
+    for (int r = 0; r < reps; ++r) launch_main_tile<T>(s, true, m, ext, bx, by, s->stream);
+    HIP_TRY(hipEventRecord(e1, s->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      best_bx = bx;
+      best_by = by;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (auto& x : saved) {
+    HIP_TRY(hipMemcpy(x.second.first, x.first, x.second.second, hipMemcpyDeviceToDevice));
+    HIP_TRY(hipFree(x.first));
+  }
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_cache[key] = {best_bx, best_by};
   return GMES_OK;
 }
 
@@ -1005,6 +1143,27 @@ int gmes_sim_set_tile(gmes_sim* s, int32_t bx, int32_t by) {
   return GMES_OK;
 }
 
+int gmes_sim_autotune(gmes_sim* s, int32_t* bx, int32_t* by) {
+  if (!s->committed) return set_error(GMES_EINVAL, "sim is not committed");
+  HIP_TRY(hipSetDevice(s->device));
+  int tbx = 0, tby = 0, rc;
+  switch (s->g.dtype) {
+    case GMES_F32: rc = autotune_t<float>(s, tbx, tby); break;
+    case GMES_F64: rc = autotune_t<double>(s, tbx, tby); break;
+    case GMES_C64: rc = autotune_t<cplx<float>>(s, tbx, tby); break;
+    default: rc = autotune_t<cplx<double>>(s, tbx, tby); break;
+  }
+  if (rc) return rc;
+  if (tbx > 0) {
+    s->tile_bx = tbx;
+    s->tile_by = tby;
+    destroy_graph(s);
+  }
+  if (bx) *bx = tbx;
+  if (by) *by = tby;
+  return GMES_OK;
+}
+
 int gmes_sim_profile(gmes_sim* s, int32_t enable) {
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(sync_exec(s));
@@ -1024,62 +1183,260 @@ int gmes_sim_read_profile(gmes_sim* s, double ms[6], int64_t launches[6]) {
 }
 
 // ----------------------------------------------------------------- batches
+}  // extern "C"
+
 struct gmes_batch {
   std::vector<gmes_sim*> sims;
-  bool graph = false;
+  int device = 0;
+  int dtype = 0;
+  hipStream_t stream = nullptr;
+  bool use_graph = true;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<uint64_t> gens;  // member generations the graph was captured with
+  // job tables: one device buffer, rebuilt on the host for every call
+  void* d_jobs = nullptr;
+  size_t jobs_cap = 0;
+  std::vector<char> h_jobs;
+  struct Launch {
+    size_t off = 0;
+    int njobs = 0;
+    int64_t nblocks = 0;
+  };
+  Launch main3d[2], mainnd[2], src[2], ghost[2];  // [0] E, [1] H
+  std::vector<int64_t> shape_sig;  // launch shapes the graph was captured with
 };
+
+namespace {
+
+size_t append_table(std::vector<char>& h, const void* p, size_t bytes) {
+  const size_t off = (h.size() + 255) / 256 * 256;
+  h.resize(off + bytes);
+  if (bytes) memcpy(h.data() + off, p, bytes);
+  return off;
+}
+
+// Job tables of every batched launch.  The tiles follow each member's own
+// setting (gmes_sim_set_tile / gmes_sim_autotune), with by = 256 / bx.
+template <class T>
+void build_jobs(gmes_batch* b) {
+  b->h_jobs.clear();
+  for (int e = 0; e < 2; ++e) {
+    const bool electric = e == 0;
+    std::vector<MainJob<T>> j3, jn;
+    int64_t nb3 = 0, nbn = 0;
+    std::vector<SrcJob<T>> js;
+    int64_t nbs = 0;
+    std::vector<GhostJob<T>> jg;
+    int64_t nbg = 0;
+    for (gmes_sim* s : b->sims) {
+      MainJob<T> J{};
+      int ext[3];
+      if (main_args<T>(s, electric, J.m, ext)) {
+        int bx, by;
+        tile_of(s, ext, bx, by);
+        J.bx = bx;
+        const int tby = 256 / bx;
+        if (is3d(s)) {
+          J.gx = (s->g.nz + bx - 1) / bx;
+          J.gy = (s->g.ny + tby - 1) / tby;
+          J.block0 = nb3;
+          nb3 += (int64_t)J.gx * J.gy * s->g.nx;
+          j3.push_back(J);
+        } else {
+          J.gx = (ext[2] + bx - 1) / bx;
+          J.gy = (ext[1] + tby - 1) / tby;
+          J.block0 = nbn;
+          nbn += (int64_t)J.gx * J.gy * ext[0];
+          jn.push_back(J);
+        }
+      }
+      SrcJob<T> S{};
+      S.a = src_args<T>(s, electric);
+      if (S.a.ngroups > 0) {
+        S.block0 = nbs;
+        nbs += (S.a.ngroups + 255) / 256;
+        js.push_back(S);
+      }
+      GhostJob<T> G{};
+      int64_t total = 0;
+      G.g = ghost_args<T>(s, electric, 0, total);
+      G.block0 = nbg;
+      nbg += std::max<int64_t>(1, (total + 255) / 256);  // >= 1: the block that ticks the clock
+      jg.push_back(G);
+    }
+    b->main3d[e] = {append_table(b->h_jobs, j3.data(), j3.size() * sizeof(MainJob<T>)), (int)j3.size(), nb3};
+    b->mainnd[e] = {append_table(b->h_jobs, jn.data(), jn.size() * sizeof(MainJob<T>)), (int)jn.size(), nbn};
+    b->src[e] = {append_table(b->h_jobs, js.data(), js.size() * sizeof(SrcJob<T>)), (int)js.size(), nbs};
+    b->ghost[e] = {append_table(b->h_jobs, jg.data(), jg.size() * sizeof(GhostJob<T>)), (int)jg.size(), nbg};
+  }
+}
+
+template <class J>
+const J* job_ptr(const gmes_batch* b, size_t off) {
+  return reinterpret_cast<const J*>(static_cast<const char*>(b->d_jobs) + off);
+}
+
+// One step of every member, in the single-sim order, with the main, source
+// and ghost kernels of all members batched into one launch each.
+template <class T>
+void enqueue_batch_step_t(gmes_batch* b, hipStream_t st) {
+  const auto& H = b->ghost[1];
+  launch_ghost_batch<T>(job_ptr<GhostJob<T>>(b, H.off), H.njobs, H.nblocks, 1, st);
+  for (int e = 0; e < 2; ++e) {
+    const bool electric = e == 0;
+    if (!electric) {
+      const auto& G = b->ghost[0];
+      launch_ghost_batch<T>(job_ptr<GhostJob<T>>(b, G.off), G.njobs, G.nblocks, 0, st);
+    }
+    launch_main_batch<T>(electric, true, job_ptr<MainJob<T>>(b, b->main3d[e].off), b->main3d[e].njobs,
+                         b->main3d[e].nblocks, st);
+    launch_main_batch<T>(electric, false, job_ptr<MainJob<T>>(b, b->mainnd[e].off), b->mainnd[e].njobs,
+                         b->mainnd[e].nblocks, st);
+    if (electric)
+      for (gmes_sim* s : b->sims) enqueue_ade<T>(s, st);
+    launch_src_batch<T>(job_ptr<SrcJob<T>>(b, b->src[e].off), b->src[e].njobs, b->src[e].nblocks, st);
+    for (gmes_sim* s : b->sims) enqueue_probe<T>(s, electric, st);
+    if (electric)
+      for (gmes_sim* s : b->sims) enqueue_aux_steps(s, st);
+  }
+}
+
+void enqueue_batch_step(gmes_batch* b, hipStream_t st) {
+  switch (b->dtype) {
+    case GMES_F32: enqueue_batch_step_t<float>(b, st); break;
+    case GMES_F64: enqueue_batch_step_t<double>(b, st); break;
+    case GMES_C64: enqueue_batch_step_t<cplx<float>>(b, st); break;
+    default: enqueue_batch_step_t<cplx<double>>(b, st); break;
+  }
+}
+
+void destroy_batch_graph(gmes_batch* b) {
+  if (b->graph_exec) {
+    (void)hipGraphExecDestroy(b->graph_exec);
+    b->graph_exec = nullptr;
+  }
+}
+
+}  // namespace
+
+extern "C" {
 
 int gmes_batch_create(gmes_sim* const* sims, int32_t n, gmes_batch** out) {
   if (n < 1) return set_error(GMES_EINVAL, "empty batch");
-  for (int i = 0; i < n; ++i)
-    if (!sims[i] || sims[i]->device != sims[0]->device) return set_error(GMES_EINVAL, "batch members must share a device");
+  for (int i = 0; i < n; ++i) {
+    gmes_sim* s = sims[i];
+    if (!s) return set_error(GMES_EINVAL, "null batch member");
+    if (s->device != sims[0]->device) return set_error(GMES_EINVAL, "batch members must share a device");
+    if (s->g.dtype != sims[0]->g.dtype) return set_error(GMES_ETYPE, "batch members must share a storage type");
+    if (s->batch_stream) return set_error(GMES_EINVAL, "sim %d is already in a batch", i);
+    if (s->parent) return set_error(GMES_EINVAL, "an auxiliary grid cannot be a batch member");
+    if (!s->committed) return set_error(GMES_EINVAL, "sim %d is not committed", i);
+    for (int k = 0; k < i; ++k)
+      if (sims[k] == s) return set_error(GMES_EINVAL, "sim %d appears twice", i);
+  }
+  HIP_TRY(hipSetDevice(sims[0]->device));
   gmes_batch* b = new gmes_batch();
   b->sims.assign(sims, sims + n);
+  b->device = sims[0]->device;
+  b->dtype = sims[0]->g.dtype;
+  if (hipStreamCreate(&b->stream) != hipSuccess) {
+    delete b;
+    return set_error(GMES_EDEVICE, "hipStreamCreate failed");
+  }
+  for (gmes_sim* s : b->sims) {
+    (void)sync_exec(s);
+    s->batch_stream = b->stream;
+  }
   *out = b;
   return GMES_OK;
 }
 
 int gmes_batch_destroy(gmes_batch* b) {
+  if (!b) return GMES_OK;
+  (void)hipSetDevice(b->device);
+  (void)hipStreamSynchronize(b->stream);
+  destroy_batch_graph(b);
+  for (gmes_sim* s : b->sims) s->batch_stream = nullptr;
+  if (b->d_jobs) (void)hipFree(b->d_jobs);
+  (void)hipStreamDestroy(b->stream);
   delete b;
   return GMES_OK;
 }
 
 int gmes_batch_set_graph(gmes_batch* b, int32_t enable) {
-  b->graph = enable != 0;
-  for (auto* s : b->sims) gmes_sim_set_graph(s, enable);
+  b->use_graph = enable != 0;
+  destroy_batch_graph(b);
   return GMES_OK;
 }
 
-// Every member runs on its own stream; the launches of one step are issued
-// round-robin over the members so that their kernels overlap on the GPU.
+// Advance every member by nsteps: one batched launch per kernel kind and
+// half step, replayed from one captured graph per step.
 int gmes_batch_step(gmes_batch* b, int64_t nsteps) {
-  for (auto* s : b->sims) {
+  if (nsteps < 0) return set_error(GMES_EINVAL, "negative step count");
+  for (gmes_sim* s : b->sims) {
+    if (s->timing.on) return set_error(GMES_EINVAL, "kernel timing is per sim; switch it off for batch members");
     int rc = check_windows(s, nsteps);
     if (rc) return rc;
     if ((rc = ensure_probe_capacity(s, nsteps))) return rc;
-    if (!s->committed) return set_error(GMES_EINVAL, "sim is not committed");
   }
-  for (auto* s : b->sims)
-    if (s->use_graph && (s->graph_dirty || !s->graph_exec)) {
-      int rc = build_graph(s);
-      if (rc) return rc;
-    }
-  HIP_TRY(hipSetDevice(b->sims[0]->device));
-  for (int64_t t = 0; t < nsteps; ++t)
-    for (auto* s : b->sims) {
-      if (s->use_graph) HIP_TRY(hipGraphLaunch(s->graph_exec, s->stream));
-      else enqueue_step(s, s->stream);
-    }
-  HIP_TRY(hipGetLastError());
-  for (auto* s : b->sims) {
-    advance_host_clock(s, nsteps);
-    s->probe_n = s->step - s->probe_base;
+  HIP_TRY(hipSetDevice(b->device));
+  // the job tables are read by kernels still in flight: wait before rewriting
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  switch (b->dtype) {
+    case GMES_F32: build_jobs<float>(b); break;
+    case GMES_F64: build_jobs<double>(b); break;
+    case GMES_C64: build_jobs<cplx<float>>(b); break;
+    default: build_jobs<cplx<double>>(b); break;
   }
+  if (b->h_jobs.size() > b->jobs_cap) {
+    destroy_batch_graph(b);
+    if (b->d_jobs) HIP_TRY(hipFree(b->d_jobs));
+    b->d_jobs = nullptr;
+    b->jobs_cap = 0;
+    HIP_TRY(hipMalloc(&b->d_jobs, b->h_jobs.size()));
+    b->jobs_cap = b->h_jobs.size();
+  }
+  HIP_TRY(hipMemcpy(b->d_jobs, b->h_jobs.data(), b->h_jobs.size(), hipMemcpyHostToDevice));
+  // launch shapes and per-member captured arguments decide whether the
+  // graph is still valid; table contents are read at run time
+  std::vector<int64_t> sig;
+  for (int e = 0; e < 2; ++e)
+    for (const auto* L : {&b->main3d[e], &b->mainnd[e], &b->src[e], &b->ghost[e]}) {
+      sig.push_back((int64_t)L->off);
+      sig.push_back(L->njobs);
+      sig.push_back(L->nblocks);
+    }
+  std::vector<uint64_t> gens;
+  for (gmes_sim* s : b->sims) {
+    gens.push_back(s->gen);
+    for (int i = 0; i < s->naux; ++i) gens.push_back(s->aux[i]->gen);
+  }
+  if (sig != b->shape_sig || gens != b->gens) destroy_batch_graph(b);
+  b->shape_sig = sig;
+  b->gens = gens;
+  if (b->use_graph) {
+    if (!b->graph_exec) {
+      hipGraph_t graph;
+      HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_batch_step(b, b->stream);
+      HIP_TRY(hipStreamEndCapture(b->stream, &graph));
+      hipError_t e = hipGraphInstantiate(&b->graph_exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) return set_error(GMES_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    for (int64_t t = 0; t < nsteps; ++t) HIP_TRY(hipGraphLaunch(b->graph_exec, b->stream));
+  } else {
+    for (int64_t t = 0; t < nsteps; ++t) enqueue_batch_step(b, b->stream);
+    HIP_TRY(hipGetLastError());
+  }
+  for (gmes_sim* s : b->sims) advance_host_clock(s, nsteps);
   return GMES_OK;
 }
 
 int gmes_batch_sync(gmes_batch* b) {
-  for (auto* s : b->sims) {
+  HIP_TRY(hipSetDevice(b->device));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for (gmes_sim* s : b->sims) {
     int rc = gmes_sim_sync(s);
     if (rc) return rc;
   }

@@ -110,6 +110,8 @@ struct gmes_sim {
   gmes_sim* aux[8] = {};
   int32_t naux = 0;
   gmes_sim* parent = nullptr;  // the sim whose stream advances this aux sim
+  hipStream_t batch_stream = nullptr;  // stream of the batch this sim belongs to
+  uint64_t gen = 0;            // bumped whenever captured launch arguments go stale
   void* d_aux_ptrs = nullptr;  // [8][6] device pointers of aux fields
   // probes
   struct Probe { int32_t comp; int64_t cell; };

@@ -109,8 +109,38 @@ struct GhostArgs {
   Ctl* ctl;
 };
 
+// Batched-launch job tables: one entry per member sim; block0 is the
+// member's first block in the launch (entries sorted by block0).
+template <class T>
+struct MainJob {
+  MainArgs<T> m;
+  int bx;      // tile width along k (power of two); the tile is bx x (256/bx)
+  int gx, gy;  // blocks along k and j; the rest of the member's blocks run along i
+  int pad_;
+  int64_t block0;
+};
+
+template <class T>
+struct SrcJob {
+  SrcArgs<T> a;
+  int64_t block0;
+};
+
+template <class T>
+struct GhostJob {
+  GhostArgs<T> g;
+  int64_t block0;
+};
+
 template <class T>
 void launch_main(bool electric, const MainArgs<T>& m, int ext[3], int bx, int by, hipStream_t st);
+template <class T>
+void launch_main_batch(bool electric, bool three_d, const MainJob<T>* jobs, int njobs, int64_t nblocks,
+                       hipStream_t st);
+template <class T>
+void launch_src_batch(const SrcJob<T>* jobs, int njobs, int64_t nblocks, hipStream_t st);
+template <class T>
+void launch_ghost_batch(const GhostJob<T>* jobs, int njobs, int64_t nblocks, int tick, hipStream_t st);
 template <class T>
 void launch_main3d(bool electric, const MainArgs<T>& m, int bx, int by, int ichunk, hipStream_t st);
 template <class T>

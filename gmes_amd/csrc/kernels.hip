@@ -231,13 +231,10 @@ __device__ __forceinline__ void main_cell(const MainArgs<T>& m, int i, int j, in
   update_cell<C>(m, rec, flat, i, j, k, D1, D2);
 }
 
-// Generic main kernel (any mode): one thread per (i,j,k); every component
+// Generic main update (any mode): one thread per (i,j,k); every component
 // of the field type whose storage contains (i,j,k) is updated.
 template <bool E, class T>
-__global__ void __launch_bounds__(256) k_main(MainArgs<T> m) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int i = blockIdx.z;
+__device__ __forceinline__ void main_generic_cell(const MainArgs<T>& m, int i, int j, int k) {
   const FieldArgs<T>& a = m.fa;
   constexpr int C0 = E ? 0 : 3;
 #define GMES_MAIN_COMP(CC)                                                              \
@@ -247,6 +244,11 @@ __global__ void __launch_bounds__(256) k_main(MainArgs<T> m) {
   GMES_MAIN_COMP(C0 + 1)
   GMES_MAIN_COMP(C0 + 2)
 #undef GMES_MAIN_COMP
+}
+
+template <bool E, class T>
+__global__ void __launch_bounds__(256) k_main(MainArgs<T> m) {
+  main_generic_cell<E, T>(m, blockIdx.z, blockIdx.y * blockDim.y + threadIdx.y, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // 3-D main kernel (all six components active).  Thread (i,j,k) owns the
@@ -378,9 +380,9 @@ struct RecStage {
   int4 v;
   int t;
   bool use_lds;
-  __device__ __forceinline__ void load(const DevRec<R>* g, int nrec) {
-    t = threadIdx.y * blockDim.x + threadIdx.x;
-    use_lds = nrec * kChunks <= (int)(blockDim.x * blockDim.y) && nrec <= kRecLds;
+  __device__ __forceinline__ void load(const DevRec<R>* g, int nrec, int tid, int nthreads) {
+    t = tid;
+    use_lds = nrec * kChunks <= nthreads && nrec <= kRecLds;
     if (use_lds && t < nrec * kChunks) v = reinterpret_cast<const int4*>(g)[t];
   }
   __device__ __forceinline__ const DevRec<R>* commit(DevRec<R>* lds, const DevRec<R>* g, int nrec) {
@@ -391,15 +393,15 @@ struct RecStage {
   }
 };
 
+// One tile of the 3-D update: thread (k, j) of planes i0 .. i0+N-1; tid and
+// nthreads are the thread's rank in the block and the block size (the
+// record-table staging and its barrier involve every thread of the block).
 template <bool E, class T, int N>
-__global__ void __launch_bounds__(256) k_main3d(MainArgs<T> m) {
+__device__ __forceinline__ void main3d_tile(const MainArgs<T>& m, DevRec<typename real_of<T>::type>* srec, int k,
+                                            int j, int i0, int tid, int nthreads) {
   using R = typename real_of<T>::type;
-  __shared__ DevRec<R> srec[kRecLds];
   const int nx = m.n[0], ny = m.n[1], nz = m.n[2];
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
   const bool act = k < nz && j < ny;  // every thread reaches the barrier
-  const int i0 = blockIdx.z * N;
   const FieldArgs<T>& a = m.fa;
   // per component: element of (i,j,k) = B + i*P + j*W + k (pitched layout)
   int B[6], P[6], W[6];
@@ -412,7 +414,7 @@ __global__ void __launch_bounds__(256) k_main3d(MainArgs<T> m) {
   const uint16_t* idb = m.ids[E ? 1 : 4];
   const uint16_t* idc = m.ids[E ? 2 : 5];
   RecStage<R> stage;
-  stage.load(m.recs, m.nrec);
+  stage.load(m.recs, m.nrec, tid, nthreads);
   if (i0 + N <= nx) {
     Planes<T, N> q;
     T u0 = T{}, u1 = T{};
@@ -453,6 +455,51 @@ __global__ void __launch_bounds__(256) k_main3d(MainArgs<T> m) {
       }
     }
   }
+}
+
+template <bool E, class T, int N>
+__global__ void __launch_bounds__(256) k_main3d(MainArgs<T> m) {
+  __shared__ DevRec<typename real_of<T>::type> srec[kRecLds];
+  main3d_tile<E, T, N>(m, srec, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
+                       blockIdx.z * N, threadIdx.y * blockDim.x + threadIdx.x, blockDim.x * blockDim.y);
+}
+
+// ------------------------------------------------------------ batched launches
+// Many sims of one storage type advance with one launch per kernel kind:
+// a job table in device memory holds each member's arguments and the index
+// of its first block; a block finds its job by binary search on that index
+// (uniform across the block) and runs the same tile body as the single-sim
+// kernel, with the member's own tile shape.
+template <class J>
+__device__ __forceinline__ int find_job(const J* jobs, int njobs, int64_t b) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].block0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <bool E, class T>
+__global__ void __launch_bounds__(256) k_main3d_batch(const MainJob<T>* jobs, int njobs) {
+  __shared__ DevRec<typename real_of<T>::type> srec[kRecLds];
+  const MainJob<T>& J = jobs[find_job(jobs, njobs, blockIdx.x)];
+  const int local = (int)(blockIdx.x - J.block0);
+  const int bxi = local % J.gx, byi = (local / J.gx) % J.gy, bzi = local / (J.gx * J.gy);
+  const int tid = threadIdx.x;
+  const int tx = tid & (J.bx - 1), ty = tid / J.bx;
+  main3d_tile<E, T, 1>(J.m, srec, bxi * J.bx + tx, byi * (256 / J.bx) + ty, bzi, tid, 256);
+}
+
+template <bool E, class T>
+__global__ void __launch_bounds__(256) k_main_batch(const MainJob<T>* jobs, int njobs) {
+  const MainJob<T>& J = jobs[find_job(jobs, njobs, blockIdx.x)];
+  const int local = (int)(blockIdx.x - J.block0);
+  const int bxi = local % J.gx, byi = (local / J.gx) % J.gy, bzi = local / (J.gx * J.gy);
+  const int tid = threadIdx.x;
+  const int tx = tid & (J.bx - 1), ty = tid / J.bx;
+  main_generic_cell<E, T>(J.m, bzi, byi * (256 / J.bx) + ty, bxi * J.bx + tx);
 }
 
 // ----------------------------------------------------------------- ADE kernels
@@ -690,9 +737,8 @@ __global__ void __launch_bounds__(256) k_ade_dm2(AdeArgs<T> m) {
 // One thread per (component, cell) group; its ops run in table order
 // (gmes/pw_source.py:74-87,106-119 point sources; :216-486 TFSF faces).
 template <class T>
-__global__ void __launch_bounds__(256) k_src(SrcArgs<T> m) {
+__device__ __forceinline__ void src_group(const SrcArgs<T>& m, int g) {
   using R = typename real_of<T>::type;
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= m.ngroups) return;
   const SrcGroup grp = m.groups[g];
   T* F = m.fa.f[grp.comp] + grp.cell;
@@ -720,6 +766,17 @@ __global__ void __launch_bounds__(256) k_src(SrcArgs<T> m) {
 }
 
 template <class T>
+__global__ void __launch_bounds__(256) k_src(SrcArgs<T> m) {
+  src_group<T>(m, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_src_batch(const SrcJob<T>* jobs, int njobs) {
+  const SrcJob<T>& J = jobs[find_job(jobs, njobs, blockIdx.x)];
+  src_group<T>(J.a, (int)(blockIdx.x - J.block0) * 256 + threadIdx.x);
+}
+
+template <class T>
 __global__ void k_probe(ProbeArgs<T> m) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= m.n) return;
@@ -740,10 +797,7 @@ __global__ void k_probe(ProbeArgs<T> m) {
 // copies produce.  Thread 0 also advances the step clock (first kernel of
 // a step).
 template <class T>
-__global__ void __launch_bounds__(256) k_ghost(GhostArgs<T> m) {
-  using R = typename real_of<T>::type;
-  if (m.tick && blockIdx.x == 0 && threadIdx.x == 0) m.ctl->clock = m.ctl->clock + 1;
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void ghost_point(const GhostArgs<T>& m, int64_t e) {
   int t = 0;
   while (t < m.ncomp && e >= m.count[t]) { e -= m.count[t]; ++t; }
   if (t >= m.ncomp) return;
@@ -782,6 +836,21 @@ __global__ void __launch_bounds__(256) k_ghost(GhostArgs<T> m) {
     if constexpr (is_cplx<T>::value) v = cmul(m.ph[t][1], v);
   }
   f[didx(m.fa, c, ix[0], ix[1], ix[2])] = v;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_ghost(GhostArgs<T> m) {
+  if (m.tick && blockIdx.x == 0 && threadIdx.x == 0) m.ctl->clock = m.ctl->clock + 1;
+  ghost_point<T>(m, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Every member gets at least one block, whose thread 0 ticks its clock.
+template <class T>
+__global__ void __launch_bounds__(256) k_ghost_batch(const GhostJob<T>* jobs, int njobs, int tick) {
+  const GhostJob<T>& J = jobs[find_job(jobs, njobs, blockIdx.x)];
+  const int64_t local = (int64_t)blockIdx.x - J.block0;
+  if (tick && local == 0 && threadIdx.x == 0) J.g.ctl->clock = J.g.ctl->clock + 1;
+  ghost_point<T>(J.g, local * 256 + threadIdx.x);
 }
 
 // --------------------------------------------------------------- launchers
@@ -825,6 +894,32 @@ void launch_ade(int cls, int c, const AdeArgs<T>& m, hipStream_t st) {
 }
 
 template <class T>
+void launch_main_batch(bool electric, bool three_d, const MainJob<T>* jobs, int njobs, int64_t nblocks,
+                       hipStream_t st) {
+  if (njobs == 0 || nblocks == 0) return;
+  const dim3 grid((unsigned)nblocks), block(256);
+  if (three_d) {
+    if (electric) hipLaunchKernelGGL((k_main3d_batch<true, T>), grid, block, 0, st, jobs, njobs);
+    else hipLaunchKernelGGL((k_main3d_batch<false, T>), grid, block, 0, st, jobs, njobs);
+  } else {
+    if (electric) hipLaunchKernelGGL((k_main_batch<true, T>), grid, block, 0, st, jobs, njobs);
+    else hipLaunchKernelGGL((k_main_batch<false, T>), grid, block, 0, st, jobs, njobs);
+  }
+}
+
+template <class T>
+void launch_src_batch(const SrcJob<T>* jobs, int njobs, int64_t nblocks, hipStream_t st) {
+  if (njobs == 0 || nblocks == 0) return;
+  hipLaunchKernelGGL((k_src_batch<T>), dim3((unsigned)nblocks), dim3(256), 0, st, jobs, njobs);
+}
+
+template <class T>
+void launch_ghost_batch(const GhostJob<T>* jobs, int njobs, int64_t nblocks, int tick, hipStream_t st) {
+  if (njobs == 0 || nblocks == 0) return;
+  hipLaunchKernelGGL((k_ghost_batch<T>), dim3((unsigned)nblocks), dim3(256), 0, st, jobs, njobs, tick);
+}
+
+template <class T>
 void launch_src(const SrcArgs<T>& m, hipStream_t st) {
   hipLaunchKernelGGL((k_src<T>), dim3((m.ngroups + 255) / 256), dim3(256), 0, st, m);
 }
@@ -846,7 +941,10 @@ void launch_ghost(const GhostArgs<T>& m, int64_t total, hipStream_t st) {
   template void launch_ade<T>(int, int, const AdeArgs<T>&, hipStream_t);                       \
   template void launch_src<T>(const SrcArgs<T>&, hipStream_t);                                  \
   template void launch_probe<T>(const ProbeArgs<T>&, hipStream_t);                              \
-  template void launch_ghost<T>(const GhostArgs<T>&, int64_t, hipStream_t);
+  template void launch_ghost<T>(const GhostArgs<T>&, int64_t, hipStream_t);                     \
+  template void launch_main_batch<T>(bool, bool, const MainJob<T>*, int, int64_t, hipStream_t);  \
+  template void launch_src_batch<T>(const SrcJob<T>*, int, int64_t, hipStream_t);               \
+  template void launch_ghost_batch<T>(const GhostJob<T>*, int, int64_t, int, hipStream_t);
 
 GMES_INSTANTIATE(float)
 GMES_INSTANTIATE(double)

@@ -212,13 +212,25 @@ int gmes_sim_get_step(const gmes_sim* sim, int64_t* step);
 int gmes_sim_set_graph(gmes_sim* sim, int32_t enable);
 /* Launch-shape override (0 = autotuned default) for the main E/H kernels. */
 int gmes_sim_set_tile(gmes_sim* sim, int32_t bx, int32_t by);
+/* Tile autotuner: times the E main kernel under each candidate tile
+ * (fields and PML state are restored afterwards), keeps the fastest per
+ * size class for the life of the process and sets it on the sim; returns
+ * the chosen tile.  No reference counterpart: the reference's update loops
+ * have no launch shape (src/cpp23_support.hh:147-197). */
+int gmes_sim_autotune(gmes_sim* sim, int32_t* bx, int32_t* by);
 /* Kernel timing on the sim's stream: enable accumulates HIP-event time per
  * kernel family; read returns ms and launch counts for
  * [E main, H main, E ADE, sources, ghost, aux].  */
 int gmes_sim_profile(gmes_sim* sim, int32_t enable);
 int gmes_sim_read_profile(gmes_sim* sim, double ms[6], int64_t launches[6]);
 
-/* ---- batches (many sims, one launch per half step) -------------------- */
+/* ---- batches (many sims, one launch per kernel kind per half step) ------
+ * Members keep their own fields, tables and clocks; the main, source and
+ * ghost kernels of all members run as one launch each (a device job table
+ * maps blocks to members), and a step of the whole batch is one captured
+ * HIP graph.  A member is not stepped on its own while in a batch.  No
+ * reference counterpart: the reference advances one FDTD at a time
+ * (gmes/fdtd.py:1099-1122). */
 
 /* Group sims of one storage type on one device; each keeps its own fields. */
 int gmes_batch_create(gmes_sim* const* sims, int32_t n, gmes_batch** batch);

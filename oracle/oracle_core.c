@@ -463,6 +463,13 @@ int gmes_sim_sync(gmes_sim *s) {
 int gmes_sim_get_step(const gmes_sim *s, int64_t *step) { *step = s->step; return GMES_OK; }
 int gmes_sim_set_graph(gmes_sim *s, int32_t enable) { (void)s; (void)enable; return GMES_OK; }
 int gmes_sim_set_tile(gmes_sim *s, int32_t bx, int32_t by) { (void)s; (void)bx; (void)by; return GMES_OK; }
+/* no launch shapes on the CPU: the tile is reported as 0 x 0 */
+int gmes_sim_autotune(gmes_sim *s, int32_t *bx, int32_t *by) {
+  (void)s;
+  if (bx) *bx = 0;
+  if (by) *by = 0;
+  return GMES_OK;
+}
 int gmes_sim_profile(gmes_sim *s, int32_t enable) { (void)s; (void)enable; return GMES_OK; }
 int gmes_sim_read_profile(gmes_sim *s, double ms[6], int64_t launches[6]) {
   (void)s;
