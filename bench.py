@@ -9,6 +9,13 @@ reference's precision).  A "step" is one full leapfrog step (E and H half
 steps with ghosts, CPML, source) over the whole grid; fields stay resident
 in HBM.  Metric: Mcell-updates/s = cells x steps / s.
 
+Also measured, as extra fields of the same line:
+
+* ``batched``: 64 heterogeneous FP32 instances (32 2-D TMz of 128-1024
+  cells per side, 32 3-D of 64-192 cells per side; dielectric, Drude and
+  Lorentz objects; UPML or CPML) advanced together for 1000 steps by one
+  launch per kernel kind, beside the same instances stepped one at a time.
+
 Printed as one JSON line on stdout; diagnostics go to stderr.
 """
 
@@ -17,6 +24,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -90,6 +99,43 @@ def cpu_baseline(n, steps, warm):
                             "(%.1f s)" % (n, steps, warm, dt))
 
 
+def batched_leg(steps, warm=5, alone_steps=50):
+    """The 64-instance batched workload: Mcell-updates/s summed over the
+    instances, batched and one instance at a time."""
+    from gmes_amd import workloads
+    from gmes_amd.batch import Batch
+    t = time.perf_counter()
+    sims = workloads.batch_instances(precision="fp32")
+    setup = time.perf_counter() - t
+    cells = sum(int(np.prod(s.space.whole_field_size)) for s in sims)
+    nbytes = sum(kernel_bytes(s, True) + kernel_bytes(s, False) for s in sims)
+    log("batch: %d instances, %.1f M cells, setup %.1f s" % (len(sims), cells / 1e6, setup))
+    # one at a time (each on its own stream, synchronised per instance)
+    lib = sims[0]._lib
+    t0 = time.perf_counter()
+    for s in sims:
+        s._advance(alone_steps)
+        lib.sync(s._sim)
+    alone = cells * alone_steps / (time.perf_counter() - t0) / 1e6
+    b = Batch(sims)
+    b.advance(warm)
+    b.sync()
+    t0 = time.perf_counter()
+    b.advance(steps)
+    b.sync()
+    secs = time.perf_counter() - t0
+    b.close()
+    for s in sims:
+        s.close()
+    value = cells * steps / secs / 1e6
+    log("batch: %d steps in %.3f s -> %.1f Mcell/s (one at a time %.1f)" % (steps, secs, value, alone))
+    return dict(instances=len(sims), cells=cells, steps=steps, value=round(value, 1),
+                unit="Mcell-updates/s", ms_per_step=round(secs / steps * 1e3, 4),
+                hbm_gbps_algorithmic=round(nbytes * steps / secs / 1e9, 1),
+                one_at_a_time_value=round(alone, 1), one_at_a_time_steps=alone_steps,
+                setup_s=round(setup, 1), dtype="f32")
+
+
 def copy_probe():
     """A plain device-to-device copy of 256 MiB in, 256 MiB out, five times."""
     import torch
@@ -111,6 +157,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (scale reference for "
                          "the FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -176,6 +224,11 @@ def main():
         log("fp64: %d steps in %.3f s -> %.1f Mcell/s" % (args.fp64_steps, s64, fp64["value"]))
         sim64.close()
 
+    # ---- batched instances -----------------------------------------------
+    batched = None
+    if not args.no_batch and args.batch_steps > 0:
+        batched = batched_leg(args.batch_steps)
+
     # ---- CPU baseline ----------------------------------------------------
     cpu = None
     parity = None
@@ -184,7 +237,6 @@ def main():
         ref_sim, cpu = cpu_baseline(n, args.cpu_steps, warm)
         log("cpu baseline:", cpu)
         # the same steps on the GPU in FP64 must reproduce the reference
-        import numpy as np
         from gmes_amd import constant as const
         g64 = workloads.spheres3d(n, precision="fp64")
         g64._advance(warm + args.cpu_steps)
@@ -216,6 +268,7 @@ def main():
                    "graph": not args.no_graph},
         "hbm_pct": round(100 * achieved / HBM_PEAK_GBPS, 1),
         "fp64": fp64,
+        "batched": batched,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "parity": parity,

@@ -300,7 +300,7 @@ bool is3d(const gmes_sim* s) { return s->g.active == 0x3Fu; }
 void tile_of(const gmes_sim* s, const int ext[3], int& bx, int& by) {
   bx = s->tile_bx;
   by = s->tile_by;
-  if (bx > 0 && by > 0) return;
+  if (bx > 0 && by > 0 && (is3d(s) || bx <= next_pow2(ext[2]))) return;
   if (is3d(s)) {
     bx = std::min(128, next_pow2(s->g.nz));
     by = std::max(1, 256 / bx);
@@ -333,20 +333,26 @@ void enqueue_main(gmes_sim* s, bool electric, hipStream_t st) {
 }
 
 template <class T>
-void enqueue_ade(gmes_sim* s, hipStream_t st) {
+AdeArgs<T> ade_args(const gmes_sim* s, int c, int cls) {
   using R = typename real_of<T>::type;
+  const AdeList& L = s->ade[c][cls];
+  AdeArgs<T> a{};
+  a.fa = field_args<T>(s);
+  a.cell = L.cell; a.rec = L.rec; a.state = L.state; a.n = (int)L.n;
+  a.recs = static_cast<const DevRec<R>*>(s->d_recs);
+  a.coef = static_cast<const R*>(s->d_coef);
+  a.d[0] = (R)s->g.dx; a.d[1] = (R)s->g.dy; a.d[2] = (R)s->g.dz;
+  a.dt = (R)s->g.dt;
+  a.ctl = reinterpret_cast<Ctl*>(s->d_clock);
+  return a;
+}
+
+template <class T>
+void enqueue_ade(gmes_sim* s, hipStream_t st) {
   for (int c = 0; c < 3; ++c)
     for (int cls = GMES_CLS_DRUDE; cls < GMES_NCLS; ++cls) {
-      const AdeList& L = s->ade[c][cls];
-      if (L.n == 0) continue;
-      AdeArgs<T> a{};
-      a.fa = field_args<T>(s);
-      a.cell = L.cell; a.rec = L.rec; a.state = L.state; a.n = (int)L.n;
-      a.recs = static_cast<const DevRec<R>*>(s->d_recs);
-      a.coef = static_cast<const R*>(s->d_coef);
-      a.d[0] = (R)s->g.dx; a.d[1] = (R)s->g.dy; a.d[2] = (R)s->g.dz;
-      a.dt = (R)s->g.dt;
-      a.ctl = reinterpret_cast<Ctl*>(s->d_clock);
+      if (s->ade[c][cls].n == 0) continue;
+      const AdeArgs<T> a = ade_args<T>(s, c, cls);
       time_begin(s, 2, st);
       launch_ade<T>(cls, c, a, st);
       time_end(s, st);
@@ -574,6 +580,7 @@ int autotune_t(gmes_sim* s, int& best_bx, int& best_by) {
   int ext[3];
   best_bx = best_by = 0;
   if (!main_args<T>(s, true, m, ext)) return GMES_OK;
+  if (!is3d(s)) return GMES_OK;  // the 2-D/1-D tile follows each field type's rows
   char key[96];
   snprintf(key, sizeof key, "%d/%d/%x/%d/%d/%d", s->device, s->g.dtype, s->g.active, ceil_log2(ext[0]),
            ceil_log2(ext[1]), ceil_log2(ext[2]));
@@ -1203,6 +1210,7 @@ struct gmes_batch {
     int64_t nblocks = 0;
   };
   Launch main3d[2], mainnd[2], src[2], ghost[2];  // [0] E, [1] H
+  Launch ade[GMES_NCLS];                          // by dispersive class
   std::vector<int64_t> shape_sig;  // launch shapes the graph was captured with
 };
 
@@ -1269,6 +1277,21 @@ void build_jobs(gmes_batch* b) {
     b->src[e] = {append_table(b->h_jobs, js.data(), js.size() * sizeof(SrcJob<T>)), (int)js.size(), nbs};
     b->ghost[e] = {append_table(b->h_jobs, jg.data(), jg.size() * sizeof(GhostJob<T>)), (int)jg.size(), nbg};
   }
+  for (int cls = GMES_CLS_DRUDE; cls < GMES_NCLS; ++cls) {
+    std::vector<AdeJob<T>> ja;
+    int64_t nba = 0;
+    for (gmes_sim* s : b->sims)
+      for (int c = 0; c < 3; ++c) {
+        if (s->ade[c][cls].n == 0) continue;
+        AdeJob<T> A{};
+        A.a = ade_args<T>(s, c, cls);
+        A.comp = c;
+        A.block0 = nba;
+        nba += (A.a.n + 255) / 256;
+        ja.push_back(A);
+      }
+    b->ade[cls] = {append_table(b->h_jobs, ja.data(), ja.size() * sizeof(AdeJob<T>)), (int)ja.size(), nba};
+  }
 }
 
 template <class J>
@@ -1293,7 +1316,8 @@ void enqueue_batch_step_t(gmes_batch* b, hipStream_t st) {
     launch_main_batch<T>(electric, false, job_ptr<MainJob<T>>(b, b->mainnd[e].off), b->mainnd[e].njobs,
                          b->mainnd[e].nblocks, st);
     if (electric)
-      for (gmes_sim* s : b->sims) enqueue_ade<T>(s, st);
+      for (int cls = GMES_CLS_DRUDE; cls < GMES_NCLS; ++cls)
+        launch_ade_batch<T>(cls, job_ptr<AdeJob<T>>(b, b->ade[cls].off), b->ade[cls].njobs, b->ade[cls].nblocks, st);
     launch_src_batch<T>(job_ptr<SrcJob<T>>(b, b->src[e].off), b->src[e].njobs, b->src[e].nblocks, st);
     for (gmes_sim* s : b->sims) enqueue_probe<T>(s, electric, st);
     if (electric)
@@ -1406,6 +1430,11 @@ int gmes_batch_step(gmes_batch* b, int64_t nsteps) {
       sig.push_back(L->njobs);
       sig.push_back(L->nblocks);
     }
+  for (const auto& L : b->ade) {
+    sig.push_back((int64_t)L.off);
+    sig.push_back(L.njobs);
+    sig.push_back(L.nblocks);
+  }
   std::vector<uint64_t> gens;
   for (gmes_sim* s : b->sims) {
     gens.push_back(s->gen);

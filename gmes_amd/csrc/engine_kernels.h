@@ -133,12 +133,22 @@ struct GhostJob {
 };
 
 template <class T>
+struct AdeJob {
+  AdeArgs<T> a;
+  int comp;
+  int pad_;
+  int64_t block0;
+};
+
+template <class T>
 void launch_main(bool electric, const MainArgs<T>& m, int ext[3], int bx, int by, hipStream_t st);
 template <class T>
 void launch_main_batch(bool electric, bool three_d, const MainJob<T>* jobs, int njobs, int64_t nblocks,
                        hipStream_t st);
 template <class T>
 void launch_src_batch(const SrcJob<T>* jobs, int njobs, int64_t nblocks, hipStream_t st);
+template <class T>
+void launch_ade_batch(int cls, const AdeJob<T>* jobs, int njobs, int64_t nblocks, hipStream_t st);
 template <class T>
 void launch_ghost_batch(const GhostJob<T>* jobs, int njobs, int64_t nblocks, int tick, hipStream_t st);
 template <class T>

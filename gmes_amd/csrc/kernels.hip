@@ -470,16 +470,6 @@ __global__ void __launch_bounds__(256) k_main3d(MainArgs<T> m) {
 // of its first block; a block finds its job by binary search on that index
 // (uniform across the block) and runs the same tile body as the single-sim
 // kernel, with the member's own tile shape.
-template <class J>
-__device__ __forceinline__ int find_job(const J* jobs, int njobs, int64_t b) {
-  int lo = 0, hi = njobs - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (jobs[mid].block0 <= b) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
 
 template <bool E, class T>
 __global__ void __launch_bounds__(256) k_main3d_batch(const MainJob<T>* jobs, int njobs) {
@@ -502,6 +492,19 @@ __global__ void __launch_bounds__(256) k_main_batch(const MainJob<T>* jobs, int 
   main_generic_cell<E, T>(J.m, bzi, byi * (256 / J.bx) + ty, bxi * J.bx + tx);
 }
 
+// Batched launches: block -> job by binary search on the jobs' first
+// blocks (uniform across a block).
+template <class J>
+__device__ __forceinline__ int find_job(const J* jobs, int njobs, int64_t b) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].block0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 // ----------------------------------------------------------------- ADE kernels
 template <class T>
 __device__ __forceinline__ void cell_ijk(const FieldArgs<T>& a, int c, int flat, int& i, int& j, int& k) {
@@ -514,9 +517,8 @@ __device__ __forceinline__ void cell_ijk(const FieldArgs<T>& a, int c, int flat,
 
 // Drude and Lorentz poles: src/pw_drude.hh:90-101,145-150; src/pw_lorentz.hh:96-107,150-156
 template <int C, class T>
-__global__ void __launch_bounds__(256) k_ade_drude(AdeArgs<T> m) {
+__device__ __forceinline__ void ade_drude_cell(const AdeArgs<T>& m, int q) {
   using R = typename real_of<T>::type;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= m.n) return;
   const int flat = m.cell[q];
   int i, j, k;
@@ -544,11 +546,15 @@ __global__ void __launch_bounds__(256) k_ade_drude(AdeArgs<T> m) {
   *F = vadd(vadd(vscl(cc[0], cu), vscl(cc[1], sum)), vscl(cc[2], e_now));
 }
 
+template <int C, class T>
+__global__ void __launch_bounds__(256) k_ade_drude(AdeArgs<T> m) {
+  ade_drude_cell<C, T>(m, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // Drude-critical-points, ADE form: src/pw_dcp.hh:105-167,200-225
 template <int C, class T>
-__global__ void __launch_bounds__(256) k_ade_dcp(AdeArgs<T> m) {
+__device__ __forceinline__ void ade_dcp_cell(const AdeArgs<T>& m, int q) {
   using R = typename real_of<T>::type;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= m.n) return;
   const int flat = m.cell[q];
   int i, j, k;
@@ -594,14 +600,18 @@ __global__ void __launch_bounds__(256) k_ade_dcp(AdeArgs<T> m) {
   *F = e_new;
 }
 
+template <int C, class T>
+__global__ void __launch_bounds__(256) k_ade_dcp(AdeArgs<T> m) {
+  ade_dcp_cell<C, T>(m, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // Drude-critical-points, PLRC/RC form: src/pw_dcp.hh:467-518,564-586.
 // State words are complex for any field type: psi_dp (re part = psi_dp_re,
 // im part = psi_dp_im), psi_cp_re, psi_cp_im.
 template <int C, class T>
-__global__ void __launch_bounds__(256) k_ade_plrc(AdeArgs<T> m) {
+__device__ __forceinline__ void ade_plrc_cell(const AdeArgs<T>& m, int q) {
   using R = typename real_of<T>::type;
   using Z = cplx<R>;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= m.n) return;
   const int flat = m.cell[q];
   int i, j, k;
@@ -648,6 +658,11 @@ __global__ void __launch_bounds__(256) k_ade_plrc(AdeArgs<T> m) {
   *F = make_val<T>(e_new.re, e_new.im);
 }
 
+template <int C, class T>
+__global__ void __launch_bounds__(256) k_ade_plrc(AdeArgs<T> m) {
+  ade_plrc_cell<C, T>(m, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // Two-level density matrix, predictor-corrector: src/pw_dm2.hh:345-407.
 // Real fields only; uses only the D2 curl term and no eps (reference quirk).
 template <class R>
@@ -662,13 +677,12 @@ __device__ double dm2_l2(R e, const R* u, int n) {
 }
 
 template <int C, class T>
-__global__ void __launch_bounds__(256) k_ade_dm2(AdeArgs<T> m) {
+__device__ __forceinline__ void ade_dm2_cell(const AdeArgs<T>& m, int q) {
   using R = typename real_of<T>::type;
   if constexpr (is_cplx<T>::value) {
     return;
   } else {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= m.n) return;
+      if (q >= m.n) return;
     const int flat = m.cell[q];
     int i, j, k;
     cell_ijk(m.fa, C, flat, i, j, k);
@@ -731,6 +745,30 @@ __global__ void __launch_bounds__(256) k_ade_dm2(AdeArgs<T> m) {
     *F = e_new;
     for (int p = 0; p < 3 * N; ++p) st[p * m.n + q] = u_new[p];
   }
+}
+
+template <int C, class T>
+__global__ void __launch_bounds__(256) k_ade_dm2(AdeArgs<T> m) {
+  ade_dm2_cell<C, T>(m, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Dispersive cells of every member in one launch per class; the job's
+// component selects the curl terms.
+template <int CLS, class T>
+__global__ void __launch_bounds__(256) k_ade_batch(const AdeJob<T>* jobs, int njobs) {
+  const AdeJob<T>& J = jobs[find_job(jobs, njobs, blockIdx.x)];
+  const int q = (int)(blockIdx.x - J.block0) * 256 + threadIdx.x;
+#define GMES_ADE_CASES(FN)                  \
+  switch (J.comp) {                         \
+    case 0: FN<0, T>(J.a, q); break;        \
+    case 1: FN<1, T>(J.a, q); break;        \
+    default: FN<2, T>(J.a, q); break;       \
+  }
+  if constexpr (CLS == GMES_CLS_DRUDE) { GMES_ADE_CASES(ade_drude_cell) }
+  else if constexpr (CLS == GMES_CLS_DCP_ADE) { GMES_ADE_CASES(ade_dcp_cell) }
+  else if constexpr (CLS == GMES_CLS_DCP_PLRC) { GMES_ADE_CASES(ade_plrc_cell) }
+  else { GMES_ADE_CASES(ade_dm2_cell) }
+#undef GMES_ADE_CASES
 }
 
 // ---------------------------------------------------------------- sources
@@ -908,6 +946,16 @@ void launch_main_batch(bool electric, bool three_d, const MainJob<T>* jobs, int 
 }
 
 template <class T>
+void launch_ade_batch(int cls, const AdeJob<T>* jobs, int njobs, int64_t nblocks, hipStream_t st) {
+  if (njobs == 0 || nblocks == 0) return;
+  const dim3 grid((unsigned)nblocks), block(256);
+  if (cls == GMES_CLS_DRUDE) hipLaunchKernelGGL((k_ade_batch<GMES_CLS_DRUDE, T>), grid, block, 0, st, jobs, njobs);
+  else if (cls == GMES_CLS_DCP_ADE) hipLaunchKernelGGL((k_ade_batch<GMES_CLS_DCP_ADE, T>), grid, block, 0, st, jobs, njobs);
+  else if (cls == GMES_CLS_DCP_PLRC) hipLaunchKernelGGL((k_ade_batch<GMES_CLS_DCP_PLRC, T>), grid, block, 0, st, jobs, njobs);
+  else if (cls == GMES_CLS_DM2) hipLaunchKernelGGL((k_ade_batch<GMES_CLS_DM2, T>), grid, block, 0, st, jobs, njobs);
+}
+
+template <class T>
 void launch_src_batch(const SrcJob<T>* jobs, int njobs, int64_t nblocks, hipStream_t st) {
   if (njobs == 0 || nblocks == 0) return;
   hipLaunchKernelGGL((k_src_batch<T>), dim3((unsigned)nblocks), dim3(256), 0, st, jobs, njobs);
@@ -944,6 +992,7 @@ void launch_ghost(const GhostArgs<T>& m, int64_t total, hipStream_t st) {
   template void launch_ghost<T>(const GhostArgs<T>&, int64_t, hipStream_t);                     \
   template void launch_main_batch<T>(bool, bool, const MainJob<T>*, int, int64_t, hipStream_t);  \
   template void launch_src_batch<T>(const SrcJob<T>*, int, int64_t, hipStream_t);               \
+  template void launch_ade_batch<T>(int, const AdeJob<T>*, int, int64_t, hipStream_t);          \
   template void launch_ghost_batch<T>(const GhostJob<T>*, int, int64_t, int, hipStream_t);
 
 GMES_INSTANTIATE(float)

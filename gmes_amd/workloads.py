@@ -9,6 +9,13 @@
   cells, eps uniform in [1.5, 12]) outside a 12-cell CPML on all faces, a
   Gaussian-pulse (Bandpass) Ex point source at the centre and Courant
   ratio 0.5; 2000 steps.  ``scale`` shrinks every length for parity runs.
+* ``batch_instances``: 64 heterogeneous instances advanced together -- 32
+  2-D TMz of side log-uniform in 128-1024 cells and 32 3-D of side
+  log-uniform in 64-192 cells, at resolution 16; background Dielectric
+  eps in [1, 4] plus 4-12 random blocks/cylinders (2-D) or blocks/spheres
+  (3-D) of Dielectric, Drude or Lorentz material; a UPML or CPML boundary
+  of 8-16 cells; a Bandpass point source of frequency in [0.3, 1.2] with
+  fwidth f/4; Courant ratio 0.5; 1000 steps in FP32.
 """
 
 import numpy as np
@@ -18,6 +25,7 @@ from . import geometry as _geo
 from . import material as _mat
 from . import source as _src
 from .constant import Ex, Ez
+from . import pygeom as _pg
 
 
 def air2d(precision="fp64", _engine=None, verbose=False):
@@ -55,3 +63,65 @@ def spheres3d(n=256, precision="fp32", nsph=40, pml=None, seed=2, _engine=None, 
                      _engine=_engine)
     sim.init()
     return sim
+
+
+def _random_material(rng):
+    kind = rng.integers(3)
+    if kind == 0:
+        return _mat.Dielectric(float(rng.uniform(1.5, 12.0)))
+    if kind == 1:
+        return _mat.Drude(eps_inf=float(rng.uniform(1.0, 4.0)),
+                          dps=(_mat.DrudePole(omega=float(rng.uniform(1.0, 10.0)),
+                                              gamma=float(rng.uniform(0.01, 0.1))),))
+    poles = tuple(_mat.LorentzPole(amp=float(rng.uniform(0.5, 3.0)), omega=float(rng.uniform(0.2, 1.0)),
+                                   gamma=float(rng.uniform(0.01, 0.1)))
+                  for _ in range(int(rng.integers(1, 4))))
+    return _mat.Lorentz(eps_inf=float(rng.uniform(1.0, 4.0)), lps=poles)
+
+
+def batch_instance(rng, three_d, side, precision="fp32", _engine=None, res=16):
+    """One instance of the batched workload: a box of ``side`` cells per
+    axis (2-D TMz or 3-D) at resolution ``res``."""
+    L = side / float(res)
+    pml = int(rng.integers(8, 17)) / float(res)
+    inner = L / 2.0 - pml
+    size = (L, L, L) if three_d else (L, L, 0)
+    space = _geo.Cartesian(size=size, resolution=res)
+    geom = [_geo.DefaultMedium(material=_mat.Dielectric(float(rng.uniform(1.0, 4.0))))]
+    for _ in range(int(rng.integers(4, 13))):
+        ext = rng.uniform(0.1, 0.3) * 2 * inner
+        c = rng.uniform(-inner + ext / 2, inner - ext / 2, 3)
+        if not three_d:
+            c[2] = 0.0
+        mat = _random_material(rng)
+        if rng.integers(2) == 0:
+            sz = rng.uniform(0.5, 1.0, 3) * ext
+            geom.append(_pg.Block(mat, center=tuple(c), size=tuple(sz) if three_d else (sz[0], sz[1], 1)))
+        elif three_d:
+            geom.append(_pg.Sphere(mat, center=tuple(c), radius=ext / 2))
+        else:
+            geom.append(_pg.Cylinder(mat, center=tuple(c), radius=ext / 2, axis=(0, 0, 1)))
+    bound = _mat.Upml() if rng.integers(2) == 0 else _mat.Cpml()
+    geom.append(_geo.Shell(material=bound, thickness=pml))
+    f = float(rng.uniform(0.3, 1.2))
+    pos = tuple(rng.uniform(-inner / 2, inner / 2, 3) * (1, 1, 1 if three_d else 0))
+    src = [_src.PointSource(_src.Bandpass(freq=f, fwidth=f / 4), center=pos, component=Ez)]
+    cls = _fdtd.FDTD if three_d else _fdtd.TMzFDTD
+    sim = cls(space, geom, src, courant_ratio=0.5, verbose=False, precision=precision, _engine=_engine)
+    sim.init()
+    return sim
+
+
+def batch_sizes(n2d=32, n3d=32, seed=4, lo2=128, hi2=1024, lo3=64, hi3=192):
+    """(three_d, side) of every instance, in a fixed seeded order."""
+    rng = np.random.default_rng(seed)
+    out = [(False, int(round(np.exp(rng.uniform(np.log(lo2), np.log(hi2)))))) for _ in range(n2d)]
+    out += [(True, int(round(np.exp(rng.uniform(np.log(lo3), np.log(hi3)))))) for _ in range(n3d)]
+    return out
+
+
+def batch_instances(n2d=32, n3d=32, seed=4, precision="fp32", _engine=None, **sizes):
+    """The instances of the batched workload (seeded; list order = draw order)."""
+    rng = np.random.default_rng(seed + 1000)
+    return [batch_instance(rng, t, s, precision=precision, _engine=_engine)
+            for t, s in batch_sizes(n2d, n3d, seed, **sizes)]
