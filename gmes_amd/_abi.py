@@ -78,6 +78,10 @@ _SIGNATURES = {
     "gmes_sim_attach_aux": (C.c_int, [_P, _P, C.POINTER(C.c_int32)]),
     "gmes_sim_add_probe": (C.c_int, [_P, C.c_int32, C.c_int64, C.POINTER(C.c_int32)]),
     "gmes_sim_read_probe": (C.c_int, [_P, C.c_int32, _P, C.c_int64, C.POINTER(C.c_int64)]),
+    "gmes_sim_add_dft": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                   C.c_int32, C.c_int32, C.POINTER(C.c_int32)]),
+    "gmes_sim_set_dft_weights": (C.c_int, [_P, C.c_int64, C.c_int64, _P]),
+    "gmes_sim_read_dft": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
     "gmes_sim_write_field": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
     "gmes_sim_read_field": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
     "gmes_sim_state_bytes": (C.c_int, [_P, C.POINTER(C.c_int64)]),
@@ -177,6 +181,23 @@ class Library(object):
         n = C.c_int64()
         self.call("gmes_sim_read_probe", h, pid, _ptr(out), max_n, C.byref(n))
         return out[:n.value].copy()
+
+    def add_dft(self, h, comp, lo, hi, nfreq, decim=1):
+        lo_a = (C.c_int64 * 3)(*[int(v) for v in lo])
+        hi_a = (C.c_int64 * 3)(*[int(v) for v in hi])
+        i = C.c_int32()
+        self.call("gmes_sim_add_dft", h, comp, lo_a, hi_a, nfreq, decim, C.byref(i))
+        return i.value
+
+    def set_dft_weights(self, h, step0, w):
+        """w: float64 array [nsteps][nfreq][2][2]."""
+        w = np.ascontiguousarray(w, np.float64)
+        self.call("gmes_sim_set_dft_weights", h, step0, w.shape[0], _ptr(w))
+
+    def read_dft(self, h, did, nfreq, ncells):
+        out = np.empty((nfreq, ncells, 2), np.float64)
+        self.call("gmes_sim_read_dft", h, did, _ptr(out), out.size)
+        return out[..., 0] + 1j * out[..., 1]
 
     def write_field(self, h, comp, arr):
         arr = np.ascontiguousarray(arr)

@@ -397,6 +397,59 @@ void enqueue_probe(gmes_sim* s, bool electric, hipStream_t st) {
   launch_probe<T>(a, st);
 }
 
+// DFT monitor jobs of a sim, first block numbered block0.
+template <class T>
+void dft_jobs(const gmes_sim* s, std::vector<DftJob<T>>& out, int64_t& block0) {
+  for (const auto& d : s->dfts) {
+    DftJob<T> J{};
+    J.f = static_cast<T*>(s->f[d.comp]);
+    J.s1 = (int)s->shape[d.comp][1];
+    J.p = s->pitch[d.comp];
+    J.koff = s->koff[d.comp];
+    J.comp = d.comp;
+    for (int a = 0; a < 3; ++a) { J.lo[a] = (int)d.lo[a]; J.n[a] = (int)d.n[a]; }
+    J.nfreq = d.nfreq;
+    J.decim = d.decim;
+    J.electric = d.comp < 3;
+    J.ncells = d.ncells;
+    J.acc = d.acc;
+    J.w = s->d_dft_w;
+    J.ctl = reinterpret_cast<Ctl*>(s->d_clock);
+    J.block0 = block0;
+    block0 += (d.ncells + 255) / 256;
+    out.push_back(J);
+  }
+}
+
+template <class T>
+int upload_dft_jobs_t(gmes_sim* s) {
+  std::vector<DftJob<T>> jobs;
+  int64_t nb = 0;
+  dft_jobs<T>(s, jobs, nb);
+  if (s->d_dft_jobs) HIP_TRY(hipFree(s->d_dft_jobs));
+  s->d_dft_jobs = nullptr;
+  s->dft_blocks = nb;
+  if (jobs.empty()) return GMES_OK;
+  HIP_TRY(hipMalloc(&s->d_dft_jobs, jobs.size() * sizeof(DftJob<T>)));
+  HIP_TRY(hipMemcpy(s->d_dft_jobs, jobs.data(), jobs.size() * sizeof(DftJob<T>), hipMemcpyHostToDevice));
+  return GMES_OK;
+}
+
+int upload_dft_jobs(gmes_sim* s) {
+  switch (s->g.dtype) {
+    case GMES_F32: return upload_dft_jobs_t<float>(s);
+    case GMES_F64: return upload_dft_jobs_t<double>(s);
+    case GMES_C64: return upload_dft_jobs_t<cplx<float>>(s);
+    default: return upload_dft_jobs_t<cplx<double>>(s);
+  }
+}
+
+template <class T>
+void enqueue_dft(gmes_sim* s, hipStream_t st) {
+  if (s->dfts.empty()) return;
+  launch_dft_batch<T>(static_cast<const DftJob<T>*>(s->d_dft_jobs), (int)s->dfts.size(), s->dft_blocks, st);
+}
+
 void enqueue_step(gmes_sim* s, hipStream_t st);
 
 // One whole step of each auxiliary (TFSF) grid of s, between the E and H
@@ -421,6 +474,7 @@ void enqueue_step_t(gmes_sim* s, hipStream_t st) {
   enqueue_main<T>(s, false, st);
   enqueue_src<T>(s, false, st);
   enqueue_probe<T>(s, false, st);
+  enqueue_dft<T>(s, st);
 }
 
 void enqueue_step(gmes_sim* s, hipStream_t st) {
@@ -457,6 +511,10 @@ int check_windows(gmes_sim* s, int64_t n) {
     return set_error(GMES_EINVAL, "waveforms cover steps [%lld, %lld), need [%lld, %lld)",
                      (long long)s->wave_step0, (long long)(s->wave_step0 + s->wave_n),
                      (long long)s->step, (long long)(s->step + n));
+  if (!s->dfts.empty() && (s->step < s->dft_step0 || s->step + n > s->dft_step0 + s->dft_n))
+    return set_error(GMES_EINVAL, "DFT weights cover steps [%lld, %lld), need [%lld, %lld)",
+                     (long long)s->dft_step0, (long long)(s->dft_step0 + s->dft_n), (long long)s->step,
+                     (long long)(s->step + n));
   for (int i = 0; i < s->naux; ++i) {
     int rc = check_windows(s->aux[i], n);
     if (rc) return rc;
@@ -658,6 +716,8 @@ int ctl_write(gmes_sim* s) {
   c.wave_step0 = s->wave_step0;
   c.wave_n = s->wave_n;
   c.probe_base = s->probe_base;
+  c.dft_step0 = s->dft_step0;
+  c.dft_n = s->dft_n;
   HIP_TRY(hipMemcpyAsync(s->d_clock, &c, sizeof c, hipMemcpyHostToDevice, s->stream));
   HIP_TRY(sync_exec(s));
   return GMES_OK;
@@ -745,8 +805,11 @@ int gmes_sim_destroy(gmes_sim* s) {
     if (s->pml_state[c]) (void)hipFree(s->pml_state[c]);
   }
   free_ade(s);
+  for (auto& d : s->dfts)
+    if (d.acc) (void)hipFree(d.acc);
   void* bufs[] = {s->d_recs, s->d_coef, s->d_prof, s->d_prof_off, s->d_ops, s->d_groups, s->d_waves,
-                  s->d_aux_ptrs, s->d_probe_cells, s->d_probe_comps, s->d_probe_buf, s->d_clock};
+                  s->d_aux_ptrs, s->d_probe_cells, s->d_probe_comps, s->d_probe_buf, s->d_clock,
+                  s->d_dft_w, s->d_dft_jobs};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -1029,6 +1092,67 @@ int gmes_sim_read_probe(gmes_sim* s, int32_t id, double* out, int64_t max, int64
   return GMES_OK;
 }
 
+int gmes_sim_add_dft(gmes_sim* s, int32_t c, const int64_t lo[3], const int64_t hi[3], int32_t nfreq,
+                     int32_t decim, int32_t* id) {
+  if (c < 0 || c > 5 || !active(s, c)) return set_error(GMES_EINVAL, "DFT of an inactive component");
+  if (nfreq < 1 || decim < 1) return set_error(GMES_EINVAL, "nfreq and decim must be positive");
+  if (!s->dfts.empty() && nfreq != s->dft_nfreq)
+    return set_error(GMES_EINVAL, "all DFT monitors of a sim share one frequency list");
+  gmes_sim::Dft d{};
+  d.comp = c;
+  d.ncells = 1;
+  for (int a = 0; a < 3; ++a) {
+    if (lo[a] < 0 || hi[a] > s->shape[c][a] || hi[a] <= lo[a]) return set_error(GMES_ERANGE, "DFT box out of bounds");
+    d.lo[a] = lo[a];
+    d.n[a] = hi[a] - lo[a];
+    d.ncells *= d.n[a];
+  }
+  d.nfreq = nfreq;
+  d.decim = decim;
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(sync_exec(s));
+  const size_t bytes = (size_t)nfreq * d.ncells * 2 * sizeof(double);
+  HIP_TRY(hipMalloc((void**)&d.acc, bytes));
+  HIP_TRY(hipMemset(d.acc, 0, bytes));
+  s->dfts.push_back(d);
+  s->dft_nfreq = nfreq;
+  s->bytes += (int64_t)bytes;
+  *id = (int32_t)s->dfts.size() - 1;
+  destroy_graph(s);
+  return upload_dft_jobs(s);
+}
+
+int gmes_sim_set_dft_weights(gmes_sim* s, int64_t step0, int64_t nsteps, const double* w) {
+  if (s->dfts.empty()) return set_error(GMES_EINVAL, "no DFT monitor");
+  if (nsteps < 0) return set_error(GMES_EINVAL, "negative step count");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(sync_exec(s));
+  const int64_t need = std::max<int64_t>(1, nsteps * s->dft_nfreq * 4);
+  if (need > s->dft_w_cap) {
+    if (s->d_dft_w) HIP_TRY(hipFree(s->d_dft_w));
+    s->d_dft_w = nullptr;
+    HIP_TRY(hipMalloc((void**)&s->d_dft_w, (size_t)need * sizeof(double)));
+    s->dft_w_cap = need;
+    int rc = upload_dft_jobs(s);  // the jobs hold the weight pointer
+    if (rc) return rc;
+    ++s->gen;
+  }
+  if (nsteps) HIP_TRY(hipMemcpy(s->d_dft_w, w, (size_t)nsteps * s->dft_nfreq * 4 * sizeof(double), hipMemcpyHostToDevice));
+  s->dft_step0 = step0;
+  s->dft_n = nsteps;
+  return ctl_write(s);
+}
+
+int gmes_sim_read_dft(gmes_sim* s, int32_t id, double* out, int64_t count) {
+  if (id < 0 || id >= (int32_t)s->dfts.size()) return set_error(GMES_EINVAL, "bad DFT id");
+  const auto& d = s->dfts[id];
+  if (count != (int64_t)d.nfreq * d.ncells * 2) return set_error(GMES_EINVAL, "size mismatch");
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(sync_exec(s));
+  HIP_TRY(hipMemcpy(out, d.acc, (size_t)count * sizeof(double), hipMemcpyDeviceToHost));
+  return GMES_OK;
+}
+
 int gmes_sim_write_field(gmes_sim* s, int32_t c, const void* host, int64_t bytes) {
   if (c < 0 || c > 5) return set_error(GMES_EINVAL, "bad component");
   if (bytes != s->ncell[c] * s->esize) return set_error(GMES_EINVAL, "size mismatch");
@@ -1211,6 +1335,7 @@ struct gmes_batch {
   };
   Launch main3d[2], mainnd[2], src[2], ghost[2];  // [0] E, [1] H
   Launch ade[GMES_NCLS];                          // by dispersive class
+  Launch dft;
   std::vector<int64_t> shape_sig;  // launch shapes the graph was captured with
 };
 
@@ -1292,6 +1417,10 @@ void build_jobs(gmes_batch* b) {
       }
     b->ade[cls] = {append_table(b->h_jobs, ja.data(), ja.size() * sizeof(AdeJob<T>)), (int)ja.size(), nba};
   }
+  std::vector<DftJob<T>> jd;
+  int64_t nbd = 0;
+  for (gmes_sim* s : b->sims) dft_jobs<T>(s, jd, nbd);
+  b->dft = {append_table(b->h_jobs, jd.data(), jd.size() * sizeof(DftJob<T>)), (int)jd.size(), nbd};
 }
 
 template <class J>
@@ -1323,6 +1452,7 @@ void enqueue_batch_step_t(gmes_batch* b, hipStream_t st) {
     if (electric)
       for (gmes_sim* s : b->sims) enqueue_aux_steps(s, st);
   }
+  launch_dft_batch<T>(job_ptr<DftJob<T>>(b, b->dft.off), b->dft.njobs, b->dft.nblocks, st);
 }
 
 void enqueue_batch_step(gmes_batch* b, hipStream_t st) {
@@ -1435,6 +1565,9 @@ int gmes_batch_step(gmes_batch* b, int64_t nsteps) {
     sig.push_back(L.njobs);
     sig.push_back(L.nblocks);
   }
+  sig.push_back((int64_t)b->dft.off);
+  sig.push_back(b->dft.njobs);
+  sig.push_back(b->dft.nblocks);
   std::vector<uint64_t> gens;
   for (gmes_sim* s : b->sims) {
     gens.push_back(s->gen);

@@ -120,6 +120,20 @@ struct gmes_sim {
   int32_t* d_probe_comps = nullptr;
   double* d_probe_buf = nullptr;   // [nprobe][cap][2]
   int64_t probe_cap = 0, probe_base = 0, probe_n = 0;  // recorded steps since base
+  // running-DFT monitors: boxes of storage indices, FP64 accumulators
+  struct Dft {
+    int32_t comp;
+    int64_t lo[3], n[3];
+    int32_t nfreq, decim;
+    int64_t ncells;
+    double* acc;
+  };
+  std::vector<Dft> dfts;
+  double* d_dft_w = nullptr;       // [row][nfreq][2][2]
+  int64_t dft_w_cap = 0, dft_step0 = 0, dft_n = 0;
+  int32_t dft_nfreq = 0;
+  void* d_dft_jobs = nullptr;      // DftJob<T>[dfts.size()]
+  int64_t dft_blocks = 0;
   // time
   int64_t step = 0;          // steps enqueued
   int64_t* d_clock = nullptr;

@@ -15,6 +15,8 @@ struct Ctl {
   int64_t wave_step0;  // absolute step of waveform row 0
   int64_t wave_n;      // rows per waveform
   int64_t probe_base;  // absolute step of probe slot 0
+  int64_t dft_step0;   // absolute step of DFT weight row 0
+  int64_t dft_n;       // rows of DFT weights
   int32_t err;         // deferred device error (1 NaN, 2 no convergence)
   int32_t pad_;
 };
@@ -132,6 +134,22 @@ struct GhostJob {
   int64_t block0;
 };
 
+// Running-DFT monitor of one component over a box of storage indices:
+// acc[f][cell] += w[row][f][E or H] * F(cell) in FP64, on steps that are
+// multiples of decim.
+template <class T>
+struct DftJob {
+  T* f;
+  int s1, p, koff, comp;
+  int lo[3], n[3];
+  int nfreq, decim, electric, pad_;
+  int64_t ncells;
+  double* acc;      // [nfreq][ncells][re, im]
+  const double* w;  // [row][nfreq][2: E, H][re, im]
+  Ctl* ctl;
+  int64_t block0;
+};
+
 template <class T>
 struct AdeJob {
   AdeArgs<T> a;
@@ -149,6 +167,8 @@ template <class T>
 void launch_src_batch(const SrcJob<T>* jobs, int njobs, int64_t nblocks, hipStream_t st);
 template <class T>
 void launch_ade_batch(int cls, const AdeJob<T>* jobs, int njobs, int64_t nblocks, hipStream_t st);
+template <class T>
+void launch_dft_batch(const DftJob<T>* jobs, int njobs, int64_t nblocks, hipStream_t st);
 template <class T>
 void launch_ghost_batch(const GhostJob<T>* jobs, int njobs, int64_t nblocks, int tick, hipStream_t st);
 template <class T>

@@ -771,6 +771,33 @@ __global__ void __launch_bounds__(256) k_ade_batch(const AdeJob<T>* jobs, int nj
 #undef GMES_ADE_CASES
 }
 
+// ------------------------------------------------------------- running DFT
+// One thread per cell of a monitor box, at the end of a step: E components
+// hold their value at n = k + 1/2 and H components at n = k + 1, each
+// weighted by its own row entry.  FP64 accumulation.
+template <class T>
+__global__ void __launch_bounds__(256) k_dft_batch(const DftJob<T>* jobs, int njobs) {
+  const DftJob<T>& J = jobs[find_job(jobs, njobs, blockIdx.x)];
+  const int64_t q = ((int64_t)blockIdx.x - J.block0) * 256 + threadIdx.x;
+  if (q >= J.ncells) return;
+  const int64_t step = J.ctl->clock - 1;
+  if (step % J.decim) return;
+  const int64_t row = step - J.ctl->dft_step0;
+  const int k = (int)(q % J.n[2]);
+  const int j = (int)((q / J.n[2]) % J.n[1]);
+  const int i = (int)(q / ((int64_t)J.n[2] * J.n[1]));
+  const T v = J.f[kPad + ((i + J.lo[0]) * J.s1 + (j + J.lo[1])) * J.p + (k + J.lo[2]) - J.koff];
+  const double vr = (double)re_of(v), vi = (double)im_of(v);
+  const double* w = J.w + row * J.nfreq * 4 + (J.electric ? 0 : 2);
+  double* acc = J.acc + 2 * q;
+  for (int f = 0; f < J.nfreq; ++f) {
+    const double wr = w[4 * f], wi = w[4 * f + 1];
+    double* a = acc + 2 * f * J.ncells;
+    a[0] = a[0] + (wr * vr - wi * vi);
+    a[1] = a[1] + (wr * vi + wi * vr);
+  }
+}
+
 // ---------------------------------------------------------------- sources
 // One thread per (component, cell) group; its ops run in table order
 // (gmes/pw_source.py:74-87,106-119 point sources; :216-486 TFSF faces).
@@ -956,6 +983,12 @@ void launch_ade_batch(int cls, const AdeJob<T>* jobs, int njobs, int64_t nblocks
 }
 
 template <class T>
+void launch_dft_batch(const DftJob<T>* jobs, int njobs, int64_t nblocks, hipStream_t st) {
+  if (njobs == 0 || nblocks == 0) return;
+  hipLaunchKernelGGL((k_dft_batch<T>), dim3((unsigned)nblocks), dim3(256), 0, st, jobs, njobs);
+}
+
+template <class T>
 void launch_src_batch(const SrcJob<T>* jobs, int njobs, int64_t nblocks, hipStream_t st) {
   if (njobs == 0 || nblocks == 0) return;
   hipLaunchKernelGGL((k_src_batch<T>), dim3((unsigned)nblocks), dim3(256), 0, st, jobs, njobs);
@@ -993,6 +1026,7 @@ void launch_ghost(const GhostArgs<T>& m, int64_t total, hipStream_t st) {
   template void launch_main_batch<T>(bool, bool, const MainJob<T>*, int, int64_t, hipStream_t);  \
   template void launch_src_batch<T>(const SrcJob<T>*, int, int64_t, hipStream_t);               \
   template void launch_ade_batch<T>(int, const AdeJob<T>*, int, int64_t, hipStream_t);          \
+  template void launch_dft_batch<T>(const DftJob<T>*, int, int64_t, hipStream_t);               \
   template void launch_ghost_batch<T>(const GhostJob<T>*, int, int64_t, int, hipStream_t);
 
 GMES_INSTANTIATE(float)

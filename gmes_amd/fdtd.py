@@ -152,6 +152,8 @@ class FDTD(object):
         self.pw_source = {}
         self._sim = None
         self._lib = None
+        self._dft_monitors = []
+        self._dft_freqs = None
 
     # ---- setup ----------------------------------------------------------
 
@@ -229,6 +231,8 @@ class FDTD(object):
         if self._sim is not None:
             lib.sim_destroy(self._sim)
         self._sim = lib.sim_create(desc)
+        self._dft_monitors = []
+        self._dft_freqs = None
 
         if self.verbose:
             print("Mapping the piecewise material.")
@@ -350,6 +354,10 @@ class FDTD(object):
         if self._waves:
             vals = np.stack([w.sample(step0, nsteps, self.time_step.dt) for w in self._waves])
             self._lib.set_waveforms(self._sim, step0, vals)
+        if self._dft_monitors:
+            from .monitor import dft_weights
+            self._lib.set_dft_weights(self._sim, step0,
+                                      dft_weights(self._dft_freqs, step0, nsteps, self.time_step.dt))
         for aux in self._aux_fdtds:
             aux._prepare(nsteps)
 
@@ -380,6 +388,12 @@ class FDTD(object):
                 for s, v in enumerate(vals):
                     n = n_last - len(vals) + s + 1 - half
                     probe.write_value(n, v if self.cmplx else v.real)
+
+    def add_dft_flux(self, freqs, z, decimation=1):
+        """A running-DFT flux monitor on the plane z (whole x-y extent) at
+        the given frequencies; see gmes_amd.monitor.DftFlux."""
+        from .monitor import DftFlux
+        return DftFlux(self, freqs, z, decimation)
 
     def sync(self):
         """Wait for the device and raise a deferred device error."""

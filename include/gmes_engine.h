@@ -189,6 +189,20 @@ int gmes_sim_add_probe(gmes_sim* sim, int32_t comp, int64_t cell, int32_t* id);
 int gmes_sim_read_probe(gmes_sim* sim, int32_t id, double* out, int64_t max,
                         int64_t* n);
 
+/* ---- running-DFT monitors ---------------------------------------------
+ * No reference counterpart (the reference has probes only); the flux
+ * monitor of the served workload is built on these.  A monitor covers one
+ * component over the box [lo, hi) of storage indices and accumulates, in
+ * FP64, acc[f][cell] += w[row][f][E or H] * F(cell) at the end of every
+ * step that is a multiple of decim (E values are those of n = k + 1/2, H
+ * values those of n = k + 1).  All monitors of a sim share nfreq. */
+int gmes_sim_add_dft(gmes_sim* sim, int32_t comp, const int64_t lo[3], const int64_t hi[3],
+                     int32_t nfreq, int32_t decim, int32_t* id);
+/* Weights for steps [step0, step0+nsteps): w[s][f][0 = E, 1 = H][re, im]. */
+int gmes_sim_set_dft_weights(gmes_sim* sim, int64_t step0, int64_t nsteps, const double* w);
+/* out[f][cell][re, im], cells of the box in C order; count = nfreq*ncells*2. */
+int gmes_sim_read_dft(gmes_sim* sim, int32_t id, double* out, int64_t count);
+
 /* ---- fields ----------------------------------------------------------- */
 
 /* Host buffers are C-order arrays of the sim's storage type. */

@@ -99,6 +99,8 @@ int gmes_sim_destroy(gmes_sim *s) {
     for (int c = 0; c < 6; ++c)
       for (int a = 0; a < 3; ++a) free(s->prof[p][c][a]);
   for (int i = 0; i < s->nprobe; ++i) free(s->probe[i].buf);
+  for (int i = 0; i < s->ndft; ++i) free(s->dft[i].acc);
+  free(s->dft_w);
   free(s->recs); free(s->coef); free(s->ops); free(s->waves);
   free(s);
   return GMES_OK;
@@ -399,6 +401,33 @@ static void apply_sources(gmes_sim *s, int c, int64_t step) {
   }
 }
 
+/* Running DFT at the end of a step: E values are those of n = k + 1/2 and
+ * H values those of n = k + 1, each with its own weight of the row. */
+static void accumulate_dft(gmes_sim *s) {
+  if (s->ndft == 0) return;
+  const int64_t row = s->step - s->dft_step0;
+  if (row < 0 || row >= s->dft_n) {
+    s->err = ocore_set_error(GMES_ERUNTIME, "DFT weights do not cover step %lld", (long long)s->step);
+    return;
+  }
+  for (int d = 0; d < s->ndft; ++d) {
+    if (s->step % s->dft[d].decim) continue;
+    const int c = s->dft[d].comp;
+    const double *w = s->dft_w + row * s->dft_nfreq * 4 + (c < 3 ? 0 : 2);
+    for (int64_t q = 0; q < s->dft[d].ncells; ++q) {
+      const int64_t k = q % s->dft[d].n[2], j = (q / s->dft[d].n[2]) % s->dft[d].n[1];
+      const int64_t i = q / (s->dft[d].n[2] * s->dft[d].n[1]);
+      const ocv v = ocore_get(s, c, i + s->dft[d].lo[0], j + s->dft[d].lo[1], k + s->dft[d].lo[2]);
+      for (int f = 0; f < s->dft_nfreq; ++f) {
+        double *a = s->dft[d].acc + 2 * (f * s->dft[d].ncells + q);
+        const double wr = w[4 * f], wi = w[4 * f + 1];
+        a[0] = a[0] + (wr * v.re - wi * v.im);
+        a[1] = a[1] + (wr * v.im + wi * v.re);
+      }
+    }
+  }
+}
+
 static void record_probes(gmes_sim *s, int electric) {
   for (int i = 0; i < s->nprobe; ++i) {
     if ((s->probe[i].comp < 3) != electric) continue;
@@ -440,7 +469,50 @@ static int one_step(gmes_sim *s) {
     if (s->err) return s->err;
   }
   record_probes(s, 0);
+  accumulate_dft(s);
+  if (s->err) return s->err;
   s->step++;
+  return GMES_OK;
+}
+
+int gmes_sim_add_dft(gmes_sim *s, int32_t c, const int64_t lo[3], const int64_t hi[3], int32_t nfreq,
+                     int32_t decim, int32_t *id) {
+  if (c < 0 || c > 5 || !(s->g.active & (1u << c))) return ocore_set_error(GMES_EINVAL, "DFT of an inactive component");
+  if (nfreq < 1 || decim < 1) return ocore_set_error(GMES_EINVAL, "nfreq and decim must be positive");
+  if (s->ndft && nfreq != s->dft_nfreq) return ocore_set_error(GMES_EINVAL, "all DFT monitors share one frequency list");
+  if (s->ndft >= 16) return ocore_set_error(GMES_EINVAL, "too many DFT monitors");
+  int d = s->ndft;
+  s->dft[d].comp = c;
+  s->dft[d].ncells = 1;
+  for (int a = 0; a < 3; ++a) {
+    if (lo[a] < 0 || hi[a] > s->shape[c][a] || hi[a] <= lo[a]) return ocore_set_error(GMES_ERANGE, "DFT box out of bounds");
+    s->dft[d].lo[a] = lo[a];
+    s->dft[d].n[a] = hi[a] - lo[a];
+    s->dft[d].ncells *= s->dft[d].n[a];
+  }
+  s->dft[d].decim = decim;
+  s->dft[d].acc = (double *)calloc((size_t)(nfreq * s->dft[d].ncells * 2), sizeof(double));
+  s->dft_nfreq = nfreq;
+  s->ndft++;
+  *id = d;
+  return GMES_OK;
+}
+
+int gmes_sim_set_dft_weights(gmes_sim *s, int64_t step0, int64_t nsteps, const double *w) {
+  if (!s->ndft) return ocore_set_error(GMES_EINVAL, "no DFT monitor");
+  free(s->dft_w);
+  size_t cnt = (size_t)(nsteps * s->dft_nfreq * 4);
+  s->dft_w = (double *)malloc(sizeof(double) * (cnt ? cnt : 1));
+  if (cnt) memcpy(s->dft_w, w, sizeof(double) * cnt);
+  s->dft_step0 = step0;
+  s->dft_n = nsteps;
+  return GMES_OK;
+}
+
+int gmes_sim_read_dft(gmes_sim *s, int32_t id, double *out, int64_t count) {
+  if (id < 0 || id >= s->ndft) return ocore_set_error(GMES_EINVAL, "bad DFT id");
+  if (count != s->dft_nfreq * s->dft[id].ncells * 2) return ocore_set_error(GMES_EINVAL, "size mismatch");
+  memcpy(out, s->dft[id].acc, sizeof(double) * count);
   return GMES_OK;
 }
 

@@ -52,6 +52,10 @@ struct gmes_sim {
   gmes_sim *aux[8]; int32_t naux;
   struct { int32_t comp; int64_t cell; double *buf; int64_t n, cap; } probe[64];
   int32_t nprobe;
+  /* running-DFT monitors (restatement of the engine's FP64 accumulation) */
+  struct { int32_t comp; int64_t lo[3], n[3], ncells; int32_t decim; double *acc; } dft[16];
+  int32_t ndft, dft_nfreq;
+  int64_t dft_step0, dft_n; double *dft_w;
   int64_t step;                  /* completed steps                    */
   int committed;
   int err;                       /* deferred error (DM2)               */
