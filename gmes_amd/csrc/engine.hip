@@ -1493,7 +1493,11 @@ int gmes_batch_create(gmes_sim* const* sims, int32_t n, gmes_batch** out) {
   b->sims.assign(sims, sims + n);
   b->device = sims[0]->device;
   b->dtype = sims[0]->g.dtype;
-  if (hipStreamCreate(&b->stream) != hipSuccess) {
+  // non-blocking: host threads that build other sims meanwhile issue
+  // null-stream copies, which must not order against (or invalidate a
+  // graph capture on) the batch stream; ordering with the members is
+  // explicit (sync_exec waits on the batch stream)
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
     delete b;
     return set_error(GMES_EDEVICE, "hipStreamCreate failed");
   }
@@ -1501,6 +1505,9 @@ int gmes_batch_create(gmes_sim* const* sims, int32_t n, gmes_batch** out) {
     (void)sync_exec(s);
     s->batch_stream = b->stream;
   }
+  // setup work of the members issued on the null stream (zero fills, table
+  // uploads) is not ordered before the non-blocking batch stream: drain it
+  HIP_TRY(hipDeviceSynchronize());
   *out = b;
   return GMES_OK;
 }

@@ -267,58 +267,89 @@ class FDTD(object):
 
     def init_source(self):
         """Map point sources and TFSF faces to engine source operations, in
-        the reference's order (gmes/fdtd.py:505-605, gmes/pw_source.py:18-46)."""
+        the reference's order (gmes/fdtd.py:505-605, gmes/pw_source.py:18-46):
+        per component, per source type in order of first appearance, per
+        cell in order of first appearance, and per cell its ops in order
+        (the faces of a TFSF edge cell in face order)."""
         self._waves = []
         env_rows = {}
         ops = []
+
+        def wave_index(row):
+            if id(row) not in env_rows:
+                env_rows[id(row)] = len(self._waves)
+                self._waves.append(row)
+            return env_rows[id(row)]
+
         for comp in tuple(self.e_field_compnt) + tuple(self.h_field_compnt):
+            shape = self._shape[comp]
             per_type = {}
             for so in self.src_list:
+                block_getter = getattr(so, "_pw_block", None)
+                if block_getter is not None:
+                    blk = block_getter(comp, shape, self.space, self.geom_tree, self._tiles)
+                    if blk is not None:
+                        per_type.setdefault("tfsf", []).append(blk)
+                    continue
                 getter = getattr(so, "_pw_ops", None)
                 if getter is None:
                     continue
-                for idx, typ, kind, payload in getter(comp, self._shape[comp], self.space,
-                                                      self.geom_tree):
-                    table = per_type.setdefault(typ, {})
-                    if typ == "point":
-                        table[idx] = [(kind, payload)]
-                    else:
-                        if idx in table and table[idx][0][1]["aux_owner"] is not payload["aux_owner"]:
-                            raise ValueError("incompatible transparent source parameters")
-                        faces = table.setdefault(idx, [])
-                        for j, (_, p) in enumerate(faces):
-                            if p["face"] is payload["face"]:
-                                faces[j] = (kind, payload)
-                                break
-                        else:
-                            faces.append((kind, payload))
-            shape = self._shape[comp]
+                for idx, typ, kind, payload in getter(comp, shape, self.space, self.geom_tree):
+                    per_type.setdefault(typ, {})[idx] = [(kind, payload)]
+            views = {}
             for typ, table in per_type.items():
+                if typ == "tfsf":
+                    arr, ncells = self._tfsf_ops(comp, shape, table, wave_index)
+                    ops.append(arr)
+                    views[typ] = PwMaterialView(typ, ncells)
+                    continue
                 for idx, entries in table.items():
                     cell = (idx[0] * shape[1] + idx[1]) * shape[2] + idx[2]
                     for kind, payload in entries:
-                        op = np.zeros((), _abi.SRC_OP_DTYPE)
+                        op = np.zeros(1, _abi.SRC_OP_DTYPE)
                         op["comp"] = comp.index
                         op["kind"] = kind
                         op["cell"] = cell
-                        if typ == "point":
-                            op["wave"] = len(self._waves)
-                            self._waves.append(payload)
-                        else:
-                            for key in ("aux", "aux_comp", "i0", "i1", "k", "r0", "r1"):
-                                op[key] = payload[key]
-                            env = payload.get("envelope")
-                            if env is None:
-                                op["wave"] = -1
-                            else:
-                                if id(env) not in env_rows:
-                                    env_rows[id(env)] = len(self._waves)
-                                    self._waves.append(env)
-                                op["wave"] = env_rows[id(env)]
+                        op["wave"] = len(self._waves)
+                        self._waves.append(payload)
                         ops.append(op)
-            self.pw_source[comp] = {t: PwMaterialView(t, len(tb)) for t, tb in per_type.items()}
-        arr = np.array(ops, _abi.SRC_OP_DTYPE) if ops else np.zeros(0, _abi.SRC_OP_DTYPE)
+                views[typ] = PwMaterialView(typ, len(table))
+            self.pw_source[comp] = views
+        arr = np.concatenate(ops) if ops else np.zeros(0, _abi.SRC_OP_DTYPE)
         self._lib.set_sources(self._sim, arr, len(self._waves))
+
+    @staticmethod
+    def _tfsf_ops(comp, shape, blocks, wave_index):
+        """Op table of the TFSF blocks of one component.  A cell on two
+        faces of one source keeps both ops, in face order, under its first
+        appearance; two sources may not share a cell (gmes/pw_source.py:
+        28-35,134-147: merging different auxiliary grids is an error)."""
+        cells, seen = [], None
+        for blk in blocks:
+            idx = blk["idx"]
+            flat = (idx[:, 0] * shape[1] + idx[:, 1]) * shape[2] + idx[:, 2]
+            u = np.unique(flat)
+            if seen is not None and np.intersect1d(seen, u).size:
+                raise ValueError("incompatible transparent source parameters")
+            seen = u if seen is None else np.union1d(seen, u)
+            cells.append(flat)
+        out, ncells = [], 0
+        for blk, flat in zip(blocks, cells):
+            _, first, inverse = np.unique(flat, return_index=True, return_inverse=True)
+            order = np.lexsort((np.arange(len(flat)), first[inverse]))
+            op = np.zeros(len(flat), _abi.SRC_OP_DTYPE)
+            op["comp"] = comp.index
+            op["kind"] = blk["kind"][order]
+            op["cell"] = flat[order]
+            env = blk["envelope"]
+            op["wave"] = -1 if env is None else wave_index(env)
+            op["aux"] = blk["aux"]
+            op["aux_comp"] = blk["aux_comp"]
+            for key in ("i0", "i1", "k", "r0", "r1"):
+                op[key] = blk[key][order]
+            out.append(op)
+            ncells += len(first)
+        return np.concatenate(out), ncells
 
     # ---- host mirrors ---------------------------------------------------
 

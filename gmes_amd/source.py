@@ -387,9 +387,27 @@ class TotalFieldScatteredField(Src):
     def _envelope_row(self, electric):
         return None
 
-    def _pw_ops(self, comp, field_shape, space, geom_tree):
-        """TFSF face corrections on ``comp`` (gmes/source.py:632-703 and the
-        face methods), in the reference's face order."""
+    def _mode_values(self, pts):
+        """mode_function at each row of pts (1 for a plane wave)."""
+        return np.ones(len(pts))
+
+    def _beam_metric(self, pts):
+        """dot(k, p - center) for each row of pts, with the reference's
+        np.dot per point (gmes/source.py:449-459); for an axis-aligned k
+        the product is exact and is taken vectorised."""
+        nz = np.flatnonzero(self.k)
+        if len(nz) == 1:
+            a = int(nz[0])
+            return (pts[:, a] - self.center[a]) * self.k[a]
+        return np.array([np.dot(self.k, p - self.center) for p in pts])
+
+    def _face_blocks(self, comp, field_shape, space, geom_tree, tiles=None):
+        """TFSF face corrections on ``comp`` (gmes/source.py:632-1406,
+        gmes/pw_source.py:197-486), one block of arrays per face in the
+        reference's face order: idx (M,3), kind, i0/i1 (flat aux indices),
+        r0, r1, k.  eps/mu come from the material tiles when given (the
+        record of a non-ghost cell holds the underlying medium's value,
+        as the reference's material_of_point/underneath rule gives)."""
         faces = self._faces(comp)
         if not faces:
             return []
@@ -397,9 +415,13 @@ class TotalFieldScatteredField(Src):
         aux = self.aux_fdtd
         aux_comp = const.Hy if electric else const.Ex
         aux_shape = aux._shape[aux_comp]
-        env = self._envelope_row(electric)
         d = space.dr
-        out = []
+        own = comp.index % 3
+        lo_ok = np.array([0 if (a == own or electric) else 1 for a in range(3)])
+        hi_ok = np.array([field_shape[a] if (a == own or not electric) else field_shape[a] - 1
+                          for a in range(3)])
+        off_c = np.array(_geometry_stagger(comp))
+        blocks = []
         for (face, axis, plus, _, idx_comp, lo_off, hi_off, samp_comp, samp_off, sign), cosine \
                 in faces:
             if not 2 * space.half_size[axis] > space.dr[axis]:
@@ -414,34 +436,92 @@ class TotalFieldScatteredField(Src):
                 high = self.center + self.half_size * flip
             low_idx = np.array(space.space_to_index(idx_comp, *low)) + lo_off
             high_idx = np.array([x + 1 for x in space.space_to_index(idx_comp, *high)]) + hi_off
-            for ijk in np.ndindex(tuple(high_idx - low_idx)):
-                idx = tuple(int(v) for v in (np.array(ijk) + low_idx))
-                if not in_range(idx, field_shape, comp):
-                    continue
-                pnt = space.index_to_space(comp, *idx)
-                mat_obj, underneath = geom_tree.material_of_point(pnt)
-                src = mat_obj if underneath is None else underneath
-                eps_mu = float(src.eps_inf if electric else src.mu_inf)
-                amp = cosine * self.amp * self.mode_function(*pnt)
-                samp_pt = space.index_to_space(samp_comp, *(np.array(idx) + samp_off))
-                samp_pnt = (0, 0, self._metric_from_center_along_beam_axis(samp_pt))
-                samp_idx = aux.space.spc_to_exact_idx(aux_comp, *samp_pnt)
-                low_s = np.floor(samp_idx).astype(np.intp)
-                i0 = tuple(int(v) for v in low_s)
-                i1 = (i0[0], i0[1], i0[2] + 1)
-                r1 = samp_idx[2] - floor(samp_idx[2])
-                r0 = 1 - r1
-                for ii in (i0, i1):
-                    if not all(0 <= ii[a] < aux_shape[a] for a in range(3)):
-                        raise ValueError("TFSF sample point outside the auxiliary grid")
-                flat0 = (i0[0] * aux_shape[1] + i0[1]) * aux_shape[2] + i0[2]
-                flat1 = (i1[0] * aux_shape[1] + i1[1]) * aux_shape[2] + i1[2]
-                k_coef = dt_over(space.dt, eps_mu, d[axis]) * amp
-                payload = dict(aux=self._aux_slot, aux_comp=aux_comp.index, i0=flat0, i1=flat1,
-                               k=k_coef, r0=r0, r1=r1, face=face, aux_owner=aux, envelope=env)
-                kind = SRC_AUX_ADD if sign > 0 else SRC_AUX_SUB
-                out.append((idx, "tfsf", kind, payload))
+            # the part of the face range that is in range (non-ghost storage)
+            a0 = np.maximum(low_idx, lo_ok)
+            a1 = np.minimum(high_idx, hi_ok)
+            if np.any(a1 <= a0):
+                continue
+            grids = np.meshgrid(*[np.arange(a0[a], a1[a]) for a in range(3)], indexing="ij")
+            idx = np.stack([g.reshape(-1) for g in grids], axis=1).astype(np.intp)
+            pnt = _index_to_space_rows(space, off_c, idx)
+            if tiles is not None:
+                rec = tiles.record_array()
+                eps_mu = rec["eps_mu"][tiles.ids[comp][idx[:, 0], idx[:, 1], idx[:, 2]]]
+            else:
+                eps_mu = np.empty(len(idx))
+                for n, p_ in enumerate(pnt):
+                    mat_obj, underneath = geom_tree.material_of_point(tuple(p_))
+                    src = mat_obj if underneath is None else underneath
+                    eps_mu[n] = float(src.eps_inf if electric else src.mu_inf)
+            amp = cosine * self.amp * self._mode_values(pnt)
+            samp_pt = _index_to_space_rows(space, np.array(_geometry_stagger(samp_comp)),
+                                           idx + np.array(samp_off))
+            z = self._beam_metric(samp_pt)
+            g = (z + aux.space.half_size[2]) / aux.space.dr[2]
+            so = _geometry_stagger(aux_comp)[2]
+            samp = g - so if so != 0 else g
+            fl = np.floor(samp)
+            i0z = fl.astype(np.intp)
+            r1 = samp - fl
+            r0 = 1 - r1
+            col = (0, 0) if aux_comp in const.E_COMPONENTS else _collapsed_index(aux_comp)[:2]
+            if i0z.min() < 0 or i0z.max() + 1 >= aux_shape[2]:
+                raise ValueError("TFSF sample point outside the auxiliary grid")
+            base = (col[0] * aux_shape[1] + col[1]) * aux_shape[2]
+            blocks.append(dict(face=face, idx=idx, kind=SRC_AUX_ADD if sign > 0 else SRC_AUX_SUB,
+                               i0=base + i0z, i1=base + i0z + 1, r0=r0, r1=r1,
+                               k=dt_over(space.dt, eps_mu, d[axis]) * amp))
+        return blocks
+
+    def _pw_block(self, comp, field_shape, space, geom_tree, tiles=None):
+        """All face ops of ``comp`` as arrays, in face order (cells on two
+        faces appear twice, once per face)."""
+        blocks = self._face_blocks(comp, field_shape, space, geom_tree, tiles)
+        if not blocks:
+            return None
+        cat = lambda key: np.concatenate([b[key] for b in blocks])
+        return dict(idx=cat("idx"), kind=np.concatenate([np.full(len(b["idx"]), b["kind"], np.int32)
+                                                         for b in blocks]),
+                    i0=cat("i0"), i1=cat("i1"), r0=cat("r0"), r1=cat("r1"), k=cat("k"),
+                    aux=self._aux_slot, aux_comp=(const.Hy if comp in const.E_COMPONENTS else const.Ex).index,
+                    aux_owner=self.aux_fdtd, envelope=self._envelope_row(comp in const.E_COMPONENTS))
+
+    def _pw_ops(self, comp, field_shape, space, geom_tree, tiles=None):
+        """The face ops of ``comp`` one by one: [(idx, "tfsf", kind, payload)]."""
+        out = []
+        env = self._envelope_row(comp in const.E_COMPONENTS)
+        aux_comp = const.Hy if comp in const.E_COMPONENTS else const.Ex
+        for b in self._face_blocks(comp, field_shape, space, geom_tree, tiles):
+            for n in range(len(b["idx"])):
+                payload = dict(aux=self._aux_slot, aux_comp=aux_comp.index, i0=int(b["i0"][n]),
+                               i1=int(b["i1"][n]), k=float(b["k"][n]), r0=float(b["r0"][n]),
+                               r1=float(b["r1"][n]), face=b["face"], aux_owner=self.aux_fdtd,
+                               envelope=env)
+                out.append((tuple(int(v) for v in b["idx"][n]), "tfsf", b["kind"], payload))
         return out
+
+
+def _geometry_stagger(comp):
+    from .geometry import STAGGER
+    return STAGGER[comp]
+
+
+def _collapsed_index(comp):
+    from .geometry import COLLAPSED_INDEX
+    return COLLAPSED_INDEX[comp]
+
+
+def _index_to_space_rows(space, off, idx):
+    """index_to_space for each row of idx, with the same expression per
+    axis: i*dr - half_size, or (i + off)*dr - half_size."""
+    out = np.empty(idx.shape, np.float64)
+    gi = idx + space.general_field_size * space.my_cart_idx
+    for a in range(3):
+        if off[a] == 0:
+            out[:, a] = gi[:, a] * space.dr[a] - space.half_size[a]
+        else:
+            out[:, a] = (gi[:, a] + off[a]) * space.dr[a] - space.half_size[a]
+    return out
 
 
 def dt_over(dt, eps_mu, d):
@@ -496,6 +576,9 @@ class GaussianBeam(TotalFieldScatteredField):
     def mode_function(self, x, y, z):
         r = self._dist_from_beam_axis(x, y, z)
         return exp(-((r / self.waist) ** 2))
+
+    def _mode_values(self, pts):
+        return np.array([self.mode_function(*p) for p in pts])
 
     def _faces(self, comp):
         """The one face of ``comp`` whose normal is the directivity axis and

@@ -125,3 +125,90 @@ def batch_instances(n2d=32, n3d=32, seed=4, precision="fp32", _engine=None, **si
     rng = np.random.default_rng(seed + 1000)
     return [batch_instance(rng, t, s, precision=precision, _engine=_engine)
             for t, s in batch_sizes(n2d, n3d, seed, **sizes)]
+
+
+def pc_slab(side, radius, steps=None, precision="fp32", res=16, nfreq=50, decimation=4,
+            _engine=None):
+    """One served solve: a photonic-crystal slab (eps 12, 0.5 a thick) with a
+    triangular lattice of air holes of the given radius (units of a), in a
+    cube of ``side`` cells at ``res`` cells per a; CPML (1 a) on z, Bloch
+    periodic x/y at k = 0 (complex fields); a TFSF plane wave along -z
+    (E along x) whose box is wider than the cell, so only its z faces act;
+    a running-DFT flux plane below the slab at ``nfreq`` frequencies in
+    [0.2, 0.6] c/a.  The lattice is not commensurate with the square cell,
+    so the holes are cut at the periodic seam.  Returns (sim, flux)."""
+    L = side / float(res)
+    dz = 1.0 / res
+    space = _geo.Cartesian(size=(L, L, L), resolution=res)
+    geom = [_geo.DefaultMedium(material=_mat.Dielectric()),
+            _pg.Block(_mat.Dielectric(12.0), center=(0, 0, 0), size=(2 * L, 2 * L, 0.5))]
+    a2 = np.array((0.5, np.sqrt(3) / 2))
+    n = int(np.ceil(L)) + 2
+    for u in range(-n, n + 1):
+        for v in range(-n, n + 1):
+            x, y = u * 1.0 + v * a2[0], v * a2[1]
+            if abs(x) <= L / 2 + radius and abs(y) <= L / 2 + radius:
+                geom.append(_pg.Cylinder(_mat.Dielectric(1.0), center=(x, y, 0), radius=radius, height=0.5,
+                                         axis=(0, 0, 1)))
+    geom.append(_geo.Shell(material=_mat.Cpml(), thickness=1.0, plus_x=False, minus_x=False,
+                           plus_y=False, minus_y=False))
+    # TFSF z faces between the CPML layers: low edge on a node, high edge
+    # half way between nodes (the rounding rule of the face index ranges)
+    zlo = -L / 2 + 1.25 + 0.0 * dz
+    zhi = L / 2 - 1.25 + 0.5 * dz
+    src = [_src.TotalFieldScatteredField(_src.Bandpass(freq=0.4, fwidth=0.4, s=5), center=(0, 0, (zlo + zhi) / 2),
+                                         size=(2 * L, 2 * L, zhi - zlo), direction=(0, 0, -1),
+                                         polarization=(1, 0, 0))]
+    sim = _fdtd.FDTD(space, geom, src, courant_ratio=0.5, bloch=(0, 0, 0), verbose=False,
+                     precision=precision, _engine=_engine)
+    sim.init()
+    flux = sim.add_dft_flux(np.linspace(0.2, 0.6, nfreq), z=-0.5 * (0.25 + (L / 2 - 1.25)),
+                            decimation=decimation)
+    return sim, flux
+
+
+def serve_requests(n=48, rate=20.0, seed=5, lo=64, hi=256):
+    """(arrival time, side, radius, steps) of the served stream: Poisson
+    arrivals at ``rate`` per second, side log-uniform in [lo, hi] cells,
+    hole radius uniform in [0.2, 0.4] a, steps uniform in 1500..4000."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    out = []
+    for i in range(n):
+        side = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+        out.append((float(t[i]), side, float(rng.uniform(0.2, 0.4)), int(rng.integers(1500, 4001))))
+    return out
+
+
+def pc_slab_footprint(side, nfreq=50, complex_fp32=True):
+    """Device bytes of a served slab solve, estimated before it is built:
+    six fields, six id arrays, full-grid CPML state of the six components,
+    and the four DFT boxes of the flux plane."""
+    es = 8 if complex_fp32 else 16
+    cells = float(side) ** 3
+    fields = cells * 6 * es
+    ids = cells * 6 * 2
+    cpml = cells * 6 * 2 * es
+    dft = nfreq * 6 * float(side) ** 2 * 16
+    return int(1.05 * (fields + ids + cpml + dft)) + (64 << 20)
+
+
+def serve_request_list(n=48, rate=20.0, seed=5, nfreq=50, decimation=4, precision="fp32",
+                       _engine=None, lo=64, hi=256, steps=None):
+    """The served stream as gmes_amd.serve.Request objects; each result is
+    the flux spectrum of its slab."""
+    from .serve import Request
+
+    def builder(side, radius):
+        def build():
+            sim, flux = pc_slab(side, radius, precision=precision, nfreq=nfreq, decimation=decimation,
+                                _engine=_engine)
+            return sim, flux.flux
+        return build
+
+    out = []
+    for i, (t, side, radius, nsteps) in enumerate(serve_requests(n, rate, seed, lo, hi)):
+        out.append(Request(i, builder(side, radius), nsteps if steps is None else steps, arrival=t,
+                           footprint=pc_slab_footprint(side, nfreq)))
+        out[-1].side = side
+    return out
