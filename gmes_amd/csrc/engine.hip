@@ -426,11 +426,17 @@ int upload_dft_jobs_t(gmes_sim* s) {
   std::vector<DftJob<T>> jobs;
   int64_t nb = 0;
   dft_jobs<T>(s, jobs, nb);
-  if (s->d_dft_jobs) HIP_TRY(hipFree(s->d_dft_jobs));
-  s->d_dft_jobs = nullptr;
   s->dft_blocks = nb;
   if (jobs.empty()) return GMES_OK;
-  HIP_TRY(hipMalloc(&s->d_dft_jobs, jobs.size() * sizeof(DftJob<T>)));
+  // capacity for 16 monitors: adding monitors rarely reallocates (a free
+  // would wait for every kernel on the device)
+  const int64_t need = std::max<int64_t>(16, (int64_t)jobs.size());
+  if (need > s->dft_jobs_cap) {
+    if (s->d_dft_jobs) HIP_TRY(hipFree(s->d_dft_jobs));
+    s->d_dft_jobs = nullptr;
+    HIP_TRY(hipMalloc(&s->d_dft_jobs, (size_t)need * sizeof(DftJob<T>)));
+    s->dft_jobs_cap = need;
+  }
   HIP_TRY(hipMemcpy(s->d_dft_jobs, jobs.data(), jobs.size() * sizeof(DftJob<T>), hipMemcpyHostToDevice));
   return GMES_OK;
 }
@@ -868,13 +874,15 @@ int gmes_sim_set_material_ids(gmes_sim* s, int32_t c, const uint16_t* ids, int64
   if (!s->ids[c]) {
     int rc = dev_alloc(s, (void**)&s->ids[c], (size_t)s->dsize[c] * sizeof(uint16_t));
     if (rc) return rc;
-    HIP_TRY(hipMemset(s->ids[c], 0, (size_t)s->dsize[c] * sizeof(uint16_t)));
   }
-  const size_t row = (size_t)s->shape[c][2] * sizeof(uint16_t);
-  if (s->pitch[c] == s->shape[c][2] && s->koff[c] == 0)
-    HIP_TRY(hipMemcpy(s->ids[c] + kPad, ids, (size_t)count * sizeof(uint16_t), hipMemcpyHostToDevice));
-  else HIP_TRY(hipMemcpy2D(s->ids[c] + kPad - s->koff[c], (size_t)s->pitch[c] * sizeof(uint16_t), ids, row, row,
-                      (size_t)(s->shape[c][0] * s->shape[c][1]), hipMemcpyHostToDevice));
+  // the device layout (pads included) is assembled on the host and copied
+  // in one contiguous transfer: a strided 2-D copy would wait for every
+  // kernel running on the device, including other sims' batches
+  std::vector<uint16_t> dev((size_t)s->dsize[c], 0);
+  const int64_t rows = s->shape[c][0] * s->shape[c][1], n2 = s->shape[c][2];
+  for (int64_t r = 0; r < rows; ++r)
+    memcpy(dev.data() + kPad - s->koff[c] + r * s->pitch[c], ids + r * n2, (size_t)n2 * sizeof(uint16_t));
+  HIP_TRY(hipMemcpy(s->ids[c], dev.data(), dev.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
   s->committed = false;
   return GMES_OK;
 }
@@ -901,29 +909,25 @@ int gmes_sim_commit(gmes_sim* s) {
     if (!active(s, c)) continue;
     const std::vector<uint16_t>& ids = s->host_ids[c];
     if ((int64_t)ids.size() != s->ncell[c]) return set_error(GMES_EINVAL, "ids of component %d not set", c);
-    const int64_t sy = s->shape[c][1], sz = s->shape[c][2];
+    const int64_t s0 = s->shape[c][0], sy = s->shape[c][1], sz = s->shape[c][2];
     std::vector<std::vector<int32_t>> cells(GMES_NCLS);
     std::vector<std::vector<uint16_t>> recs(GMES_NCLS);
     std::vector<int> words(GMES_NCLS, 0);
+    // records in use
+    std::vector<int64_t> count(s->recs.size(), 0);
     for (int64_t x = 0; x < s->ncell[c]; ++x) {
       const uint16_t id = ids[x];
       if (id >= s->recs.size()) return set_error(GMES_ERANGE, "material id out of range");
+      ++count[id];
+    }
+    bool any_ade = false;
+    for (size_t id = 0; id < s->recs.size(); ++id) {
+      if (!count[id]) continue;
       const gmes_mat_rec& r = s->recs[id];
       s->has_cls[c][r.cls] = true;
-      // Ghost planes must be Dummy: a kernel never reads past them
-      // (gmes/fdtd.py:365-366 and the analogous lines).
-      const int64_t ix[3] = {x / (sy * sz), (x / sz) % sy, x % sz};
-      bool ghost = false;
-      for (int a = 0; a < 3; ++a) {
-        if (a == c % 3) continue;
-        if (c < 3 ? ix[a] == s->shape[c][a] - 1 : ix[a] == 0) ghost = true;
-      }
-      if (ghost && r.cls != GMES_CLS_DUMMY)
-        return set_error(GMES_EINVAL, "component %d: ghost cell %lld is not Dummy", c, (long long)x);
       if (r.cls >= GMES_CLS_DRUDE) {
         if (c >= 3) return set_error(GMES_EINVAL, "dispersive class on a magnetic component");
-        cells[r.cls].push_back((int32_t)x);
-        recs[r.cls].push_back(id);
+        any_ade = true;
         words[r.cls] = std::max(words[r.cls], state_words(r));
       }
       if (r.cls == GMES_CLS_CPML || r.cls == GMES_CLS_UPML) {
@@ -933,6 +937,33 @@ int gmes_sim_commit(gmes_sim* s) {
             return set_error(GMES_EINVAL, "profile %d of component %d axis %d missing", r.prof, c, a);
       }
     }
+    // Ghost planes must be Dummy: a kernel never reads past them
+    // (gmes/fdtd.py:365-366 and the analogous lines).  E: the last plane
+    // of each transverse axis; H: plane 0.
+    const int64_t n3[3] = {s0, sy, sz};
+    for (int a = 0; a < 3; ++a) {
+      if (a == c % 3) continue;
+      const int64_t g = c < 3 ? n3[a] - 1 : 0;
+      int64_t ix[3];
+      for (ix[0] = 0; ix[0] < s0; ++ix[0])
+        for (ix[1] = 0; ix[1] < sy; ++ix[1]) {
+          if (a == 0 && ix[0] != g) break;
+          if (a == 1 && ix[1] != g) continue;
+          for (ix[2] = (a == 2 ? g : 0); ix[2] < (a == 2 ? g + 1 : sz); ++ix[2]) {
+            const int64_t x = (ix[0] * sy + ix[1]) * sz + ix[2];
+            if (s->recs[ids[x]].cls != GMES_CLS_DUMMY)
+              return set_error(GMES_EINVAL, "component %d: ghost cell %lld is not Dummy", c, (long long)x);
+          }
+        }
+    }
+    if (any_ade)
+      for (int64_t x = 0; x < s->ncell[c]; ++x) {
+        const gmes_mat_rec& r = s->recs[ids[x]];
+        if (r.cls >= GMES_CLS_DRUDE) {
+          cells[r.cls].push_back((int32_t)x);
+          recs[r.cls].push_back(ids[x]);
+        }
+      }
     // full-grid CPML (2 words) / UPML (1 word) state
     const int pw = s->has_cls[c][GMES_CLS_CPML] ? 2 : (s->has_cls[c][GMES_CLS_UPML] ? 1 : 0);
     if (s->pml_state[c]) { dev_free(s, s->pml_state[c], 0); s->pml_state[c] = nullptr; }

@@ -132,7 +132,8 @@ struct gmes_sim {
   double* d_dft_w = nullptr;       // [row][nfreq][2][2]
   int64_t dft_w_cap = 0, dft_step0 = 0, dft_n = 0;
   int32_t dft_nfreq = 0;
-  void* d_dft_jobs = nullptr;      // DftJob<T>[dfts.size()]
+  void* d_dft_jobs = nullptr;      // DftJob<T>[dft_jobs_cap]
+  int64_t dft_jobs_cap = 0;
   int64_t dft_blocks = 0;
   // time
   int64_t step = 0;          // steps enqueued

@@ -198,9 +198,16 @@ class FDTD(object):
             src = self.space.index_to_space(comp, *last)[axis]
         return cexp(1j * self.bloch[axis] * (dest - src))
 
-    def init(self):
+    def host_tables(self):
+        """The material tables of this simulation, built on the host (the
+        costly part of init(); may run in another process)."""
+        return build_tiles(self.space, self.geom_list, self.e_field_compnt, self.h_field_compnt,
+                           self.cmplx)
+
+    def init(self, tiles=None):
         """Allocate fields, map materials and sources, build the device sim
-        (gmes/fdtd.py:227-278)."""
+        (gmes/fdtd.py:227-278).  ``tiles``: the result of host_tables(),
+        when it was built elsewhere."""
         st = datetime.now()
         lib = self._engine_override if self._engine_override is not None else _abi.engine()
         self._lib = lib
@@ -236,8 +243,8 @@ class FDTD(object):
 
         if self.verbose:
             print("Mapping the piecewise material.")
-        tiles = build_tiles(self.space, self.geom_list, self.e_field_compnt,
-                            self.h_field_compnt, self.cmplx)
+        if tiles is None:
+            tiles = self.host_tables()
         self._tiles = tiles
         lib.set_materials(self._sim, tiles.record_array(), tiles.coef_array())
         for c in comps:

@@ -63,6 +63,10 @@ class Server(object):
         self.peak_reserved = 0
         self.peak_active = 0
         self.rounds = 0
+        # host-side accounting (seconds): waiting with nothing to run,
+        # forming batches, enqueueing rounds, waiting for finished rounds
+        self.t_idle = self.t_batch = self.t_advance = self.t_sync = 0.0
+        self.rebatches = 0
 
     def run(self, requests):
         """Serve ``requests`` (arrival times relative to the start of the
@@ -109,25 +113,34 @@ class Server(object):
                         active.append(r)
                         changed = True
                 if not active:
+                    ti = now()
                     if building:
                         next(iter(building.values())).exception()  # wait for a build
                     elif pending:
                         time.sleep(max(0.0, min(0.05, pending[0].arrival - now())))
+                    self.t_idle += now() - ti
                     continue
                 if changed:
+                    tb = now()
                     if batch is not None:
                         batch.close()
                     batch = Batch([r.sim for r in active], autotune=self.autotune)
                     changed = False
+                    self.rebatches += 1
+                    self.t_batch += now() - tb
                 self.peak_active = max(self.peak_active, len(active))
                 n = min(self.chunk, min(r.remaining for r in active))
+                ta = now()
                 batch.advance(n)
+                self.t_advance += now() - ta
                 self.rounds += 1
                 for r in active:
                     r.remaining -= n
                 finished = [r for r in active if r.remaining == 0]
                 if finished:
+                    ts = now()
                     batch.sync()
+                    self.t_sync += now() - ts
                     for r in finished:
                         r.result = r.result_fn()
                         r.t_done = now()

@@ -127,8 +127,15 @@ def batch_instances(n2d=32, n3d=32, seed=4, precision="fp32", _engine=None, **si
             for t, s in batch_sizes(n2d, n3d, seed, **sizes)]
 
 
+def _tables_of(sim_args):
+    """Material tables of a slab, for a worker process (host only)."""
+    side, radius, res, precision = sim_args
+    sim = pc_slab(side, radius, precision=precision, res=res, _engine=False)
+    return sim.host_tables()
+
+
 def pc_slab(side, radius, steps=None, precision="fp32", res=16, nfreq=50, decimation=4,
-            _engine=None):
+            _engine=None, tiles=None):
     """One served solve: a photonic-crystal slab (eps 12, 0.5 a thick) with a
     triangular lattice of air holes of the given radius (units of a), in a
     cube of ``side`` cells at ``res`` cells per a; CPML (1 a) on z, Bloch
@@ -159,9 +166,13 @@ def pc_slab(side, radius, steps=None, precision="fp32", res=16, nfreq=50, decima
     src = [_src.TotalFieldScatteredField(_src.Bandpass(freq=0.4, fwidth=0.4, s=5), center=(0, 0, (zlo + zhi) / 2),
                                          size=(2 * L, 2 * L, zhi - zlo), direction=(0, 0, -1),
                                          polarization=(1, 0, 0))]
+    if _engine is False:
+        # host tables only: no device, no source set-up
+        return _fdtd.FDTD(space, geom, [], courant_ratio=0.5, bloch=(0, 0, 0), verbose=False,
+                          precision=precision)
     sim = _fdtd.FDTD(space, geom, src, courant_ratio=0.5, bloch=(0, 0, 0), verbose=False,
                      precision=precision, _engine=_engine)
-    sim.init()
+    sim.init(tiles=tiles)
     flux = sim.add_dft_flux(np.linspace(0.2, 0.6, nfreq), z=-0.5 * (0.25 + (L / 2 - 1.25)),
                             decimation=decimation)
     return sim, flux
@@ -194,15 +205,19 @@ def pc_slab_footprint(side, nfreq=50, complex_fp32=True):
 
 
 def serve_request_list(n=48, rate=20.0, seed=5, nfreq=50, decimation=4, precision="fp32",
-                       _engine=None, lo=64, hi=256, steps=None):
+                       _engine=None, lo=64, hi=256, steps=None, host_pool=None):
     """The served stream as gmes_amd.serve.Request objects; each result is
-    the flux spectrum of its slab."""
+    the flux spectrum of its slab.  host_pool: an executor (processes) that
+    builds the material tables, so that set-up runs in parallel."""
     from .serve import Request
 
     def builder(side, radius):
         def build():
+            tiles = None
+            if host_pool is not None:
+                tiles = host_pool.submit(_tables_of, (side, radius, 16, precision)).result()
             sim, flux = pc_slab(side, radius, precision=precision, nfreq=nfreq, decimation=decimation,
-                                _engine=_engine)
+                                _engine=_engine, tiles=tiles)
             return sim, flux.flux
         return build
 

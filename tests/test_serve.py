@@ -96,3 +96,16 @@ def test_served_slabs_equal_alone_gpu(engine_lib):
         got = [d for d in done if d.rid == r.rid][0].result
         np.testing.assert_array_equal(got, want)
         assert np.all(np.isfinite(got)) and np.any(got != 0)
+
+
+def test_tables_built_in_another_process_are_identical(oracle_lib):
+    """The served slab's material tables built by the host-only builder
+    (as a worker process does) give the same run as tables built in place."""
+    tiles = workloads._tables_of((64, 0.3, 16, "fp64"))
+    a, fa = workloads.pc_slab(64, 0.3, precision="fp64", nfreq=4, _engine=oracle_lib, tiles=tiles)
+    b, fb = workloads.pc_slab(64, 0.3, precision="fp64", nfreq=4, _engine=oracle_lib)
+    a._advance(12)
+    b._advance(12)
+    for c in g.constant.COMPONENTS:
+        np.testing.assert_array_equal(np.asarray(a.field[c]), np.asarray(b.field[c]))
+    np.testing.assert_array_equal(fa.flux(), fb.flux())
