@@ -15,6 +15,9 @@ Also measured, as extra fields of the same line:
   cells per side, 32 3-D of 64-192 cells per side; dielectric, Drude and
   Lorentz objects; UPML or CPML) advanced together for 1000 steps by one
   launch per kernel kind, beside the same instances stepped one at a time.
+* ``dispersive``: the 384 x 384 x 192 dispersive Bloch cell (Drude blocks,
+  Lorentz spheres, a DCP slab, CPML on z, TFSF plane wave) in complex FP32,
+  timed over a few hundred of its 3000 steps.
 * ``served``: a Poisson stream (20 requests/s) of photonic-crystal-slab
   solves (64^3-256^3 cells, complex FP32, TFSF plane wave, Bloch x/y, CPML
   on z, a 50-frequency running-DFT flux plane, 1500-4000 steps) through
@@ -142,6 +145,33 @@ def batched_leg(steps, warm=5, alone_steps=50):
                 setup_s=round(setup, 1), dtype="f32")
 
 
+def dispersive_leg(steps, warm=10):
+    """The dispersive Bloch cell in complex FP32."""
+    from gmes_amd import workloads
+    t = time.perf_counter()
+    sim = workloads.dispersive3d(precision="fp32")
+    setup = time.perf_counter() - t
+    lib = sim._lib
+    cells = int(np.prod(sim.space.whole_field_size))
+    lib.set_graph(sim._sim, 1)
+    sim._advance(warm)
+    secs = time_steps(sim, lib, steps)
+    lib.profile(sim._sim, 1)
+    sim._advance(50)
+    lib.sync(sim._sim)
+    ms, launches = lib.read_profile(sim._sim)
+    lib.profile(sim._sim, 0)
+    sim.close()
+    value = cells * steps / secs / 1e6
+    split = {k: round(ms[i] / max(1, launches[i]), 4) for i, k in enumerate(
+        ("e_main", "h_main", "ade", "sources", "ghost", "aux_grid"))}
+    log("dispersive: %d steps in %.3f s -> %.1f Mcell/s (setup %.1f s), ms/launch %s"
+        % (steps, secs, value, setup, split))
+    return dict(cells=cells, steps=steps, value=round(value, 1), unit="Mcell-updates/s",
+                ms_per_step=round(secs / steps * 1e3, 4), dtype="c64", setup_s=round(setup, 1),
+                ms_per_launch=split)
+
+
 def served_leg(nreq, budget_gb=64.0, max_active=16, chunk=250):
     """The served photonic-crystal-slab stream through gmes_amd.serve."""
     import multiprocessing
@@ -199,6 +229,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--dispersive-steps", type=int, default=400)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (scale reference for "
@@ -251,10 +282,14 @@ def main():
                     kernel="k_main<%s,float>" % ("E" if fam == 0 else "H"),
                     bytes_per_launch=int(nbytes), avg_launch_ms=round(avg_ms, 5),
                     other_kernel_gbps=round(other_gbps, 1))
+    fp32_steps_done = args.warmup + args.steps + prof_steps
+    from gmes_amd import constant as const
+    fp32_fields = {c: np.array(sim.field[c]) for c in const.COMPONENTS}
     sim.close()
 
     # ---- FP64 (the reference's precision) --------------------------------
     fp64 = None
+    fp32_err = None
     if args.fp64_steps > 0:
         sim64 = workloads.spheres3d(n, precision="fp64")
         lib.set_graph(sim64._sim, 0 if args.no_graph else 1)
@@ -263,12 +298,30 @@ def main():
         fp64 = dict(value=round(cells * args.fp64_steps / s64 / 1e6, 1), unit="Mcell-updates/s",
                     steps=args.fp64_steps, ms_per_step=round(s64 / args.fp64_steps * 1e3, 4))
         log("fp64: %d steps in %.3f s -> %.1f Mcell/s" % (args.fp64_steps, s64, fp64["value"]))
+        # FP32 error against FP64 at the same step (the FP32 run above)
+        done64 = min(args.warmup, 10) + args.fp64_steps
+        sim64._advance(fp32_steps_done - done64)
+        lib.sync(sim64._sim)
+        from gmes_amd import constant as const
+        num = den = 0.0
+        for c in const.COMPONENTS:
+            a = np.asarray(fp32_fields[c], np.float64)
+            b = np.asarray(sim64.field[c], np.float64)
+            num += float(np.sum((a - b) ** 2))
+            den += float(np.sum(b ** 2))
+        fp32_err = dict(steps=fp32_steps_done, rel_l2_fp32_vs_fp64=(num / den) ** 0.5 if den else 0.0)
+        log("fp32 vs fp64 after %d steps: %.3g" % (fp32_steps_done, fp32_err["rel_l2_fp32_vs_fp64"]))
         sim64.close()
 
     # ---- batched instances -----------------------------------------------
     batched = None
     if not args.no_batch and args.batch_steps > 0:
         batched = batched_leg(args.batch_steps)
+
+    # ---- dispersive Bloch cell --------------------------------------------
+    dispersive = None
+    if args.dispersive_steps > 0:
+        dispersive = dispersive_leg(args.dispersive_steps)
 
     # ---- served stream ---------------------------------------------------
     served = None
@@ -315,10 +368,12 @@ def main():
         "hbm_pct": round(100 * achieved / HBM_PEAK_GBPS, 1),
         "fp64": fp64,
         "batched": batched,
+        "dispersive": dispersive,
         "served": served,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "parity": parity,
+        "fp32_error": fp32_err,
     }
     print(json.dumps(out), flush=True)
 

@@ -227,3 +227,54 @@ def serve_request_list(n=48, rate=20.0, seed=5, nfreq=50, decimation=4, precisio
                            footprint=pc_slab_footprint(side, nfreq)))
         out[-1].side = side
     return out
+
+
+def dispersive3d(scale=1.0, seed=3, precision="fp32", _engine=None, courant=0.3, src_time=None):
+    """3-D dispersive Bloch cell: 384 x 384 x 192 cells (times ``scale``) at
+    dx = 1; a periodic 8 x 8 array of Drude blocks (24 cells, pitch 48;
+    wp in [1, 10], gamma in [0.01, 0.1]), 16 seed-placed Lorentz spheres
+    (radius 4-16 cells, 1-3 poles), a 16-cell two-critical-point DCP-ADE
+    slab, a 10-cell CPML on z, Bloch-periodic complex fields on x/y at a
+    random k, and a Bandpass plane wave along -z (E along x) through a
+    TFSF box that spans x/y (only its z faces act).  Courant 0.3: the
+    reference's Drude recursion grows without bound at dx = 1 for
+    wp * dt above about 2 (wp = 10 at Courant 0.5), and 0.3 keeps the
+    whole wp range stable without changing the method."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = int(round(384 * scale)), int(round(384 * scale)), int(round(192 * scale))
+    space = _geo.Cartesian(size=(nx, ny, nz), resolution=1)
+    kx, ky = rng.uniform(-np.pi / 384, np.pi / 384, 2)
+    geom = [_geo.DefaultMedium(material=_mat.Dielectric())]
+    pitch, blk = 48 * scale, 24 * scale
+    nb = max(1, int(round(8 * scale * 384 / 384)))
+    for u in range(nb):
+        for v in range(nb):
+            c = (-nx / 2 + (u + 0.5) * pitch, -ny / 2 + (v + 0.5) * pitch, 0.0)
+            mat = _mat.Drude(eps_inf=1.0, dps=(_mat.DrudePole(omega=float(rng.uniform(1, 10)),
+                                                              gamma=float(rng.uniform(0.01, 0.1))),))
+            geom.append(_pg.Block(mat, center=c, size=(blk, blk, blk)))
+    zin = nz / 2 - 10 * max(scale, 0.5) - 4
+    for _ in range(16):
+        r = float(rng.uniform(4, 16) * scale)
+        c = (float(rng.uniform(-nx / 2, nx / 2)), float(rng.uniform(-ny / 2, ny / 2)),
+             float(rng.uniform(-zin + r, zin - r)))
+        poles = tuple(_mat.LorentzPole(amp=float(rng.uniform(0.5, 3.0)), omega=float(rng.uniform(0.2, 1.0)),
+                                       gamma=float(rng.uniform(0.01, 0.1)))
+                      for _ in range(int(rng.integers(1, 4))))
+        geom.append(_pg.Sphere(_mat.Lorentz(eps_inf=float(rng.uniform(1.0, 3.0)), lps=poles), center=c,
+                               radius=r))
+    dcp = _mat.DcpAde(eps_inf=1.5, sigma=0.01, dps=(_mat.DrudePole(omega=1.2, gamma=0.05),),
+                      cps=(_mat.CriticalPoint(amp=0.8, phi=-0.4, omega=0.6, gamma=0.1),
+                           _mat.CriticalPoint(amp=0.4, phi=0.7, omega=0.9, gamma=0.2)))
+    geom.append(_pg.Block(dcp, center=(0, 0, -zin + 8 * scale + 4), size=(2 * nx, 2 * ny, 16 * scale)))
+    geom.append(_geo.Shell(material=_mat.Cpml(), thickness=max(1.0, round(10 * scale)), plus_x=False,
+                           minus_x=False, plus_y=False, minus_y=False))
+    ztop = zin + 2
+    st = _src.Bandpass(freq=0.05, fwidth=0.025) if src_time is None else src_time
+    src = [_src.TotalFieldScatteredField(st, center=(0, 0, 0.25),
+                                         size=(2 * nx, 2 * ny, 2 * ztop - 0.5), direction=(0, 0, -1),
+                                         polarization=(1, 0, 0))]
+    sim = _fdtd.FDTD(space, geom, src, courant_ratio=courant, bloch=(kx, ky, 0.0), verbose=False,
+                     precision=precision, _engine=_engine)
+    sim.init()
+    return sim

@@ -213,3 +213,27 @@ def test_tfsf_scatterer_full_run(engine_lib, oracle_lib):
     gpu = fields(sim)
     assert rel_l2(gpu, fields(ref)) <= FP64_TOL
     assert abs(np.abs(gpu[g.Ez]).max() - 2.0727) / 2.0727 < 2e-4
+
+
+def _c3(lib, precision="fp64"):
+    from gmes_amd import workloads
+    return workloads.dispersive3d(scale=0.125, precision=precision, _engine=lib,
+                                  src_time=g.Bandpass(freq=0.05, fwidth=0.1, s=3))
+
+
+def test_dispersive_bloch_cell_c128(engine_lib, oracle_lib):
+    """The dispersive Bloch cell (Drude blocks, Lorentz spheres, DCP slab,
+    CPML on z, TFSF) at 1/8 scale, complex128, against the oracle."""
+    a, b = _c3(engine_lib), _c3(oracle_lib)
+    a._advance(80)
+    b._advance(80)
+    fa, fb = fields(a), fields(b)
+    assert max(np.abs(v).max() for v in fb.values()) > 1e-3
+    assert rel_l2(fa, fb) <= FP64_TOL
+
+
+def test_dispersive_bloch_cell_c64(engine_lib):
+    a, b = _c3(engine_lib, "fp32"), _c3(engine_lib)
+    a._advance(80)
+    b._advance(80)
+    assert rel_l2(fields(a), fields(b)) <= 1e-5
