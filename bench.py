@@ -226,6 +226,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-ghost-fusion", action="store_true",
+                    help="keep the separate E ghost-plane copy in the 256^3 legs")
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=48)
@@ -254,6 +256,7 @@ def main():
     log("fp32 setup %.1f s, footprint %.2f GB" % (time.perf_counter() - t,
                                                   lib.footprint(sim._sim) / 1e9))
     lib.set_graph(sim._sim, 0 if args.no_graph else 1)
+    fused = lib.set_ghost_fusion(sim._sim, 0 if args.no_ghost_fusion else 1)
     sim._advance(args.warmup)
     secs = time_steps(sim, lib, args.steps)
     value = cells * args.steps / secs / 1e6
@@ -293,6 +296,7 @@ def main():
     if args.fp64_steps > 0:
         sim64 = workloads.spheres3d(n, precision="fp64")
         lib.set_graph(sim64._sim, 0 if args.no_graph else 1)
+        lib.set_ghost_fusion(sim64._sim, 0 if args.no_ghost_fusion else 1)
         sim64._advance(min(args.warmup, 10))
         s64 = time_steps(sim64, lib, args.fp64_steps)
         fp64 = dict(value=round(cells * args.fp64_steps / s64 / 1e6, 1), unit="Mcell-updates/s",
@@ -364,7 +368,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "3-D FDTD %d^3, 40 dielectric spheres, 12-cell CPML, Bandpass Ex "
                                "source, Courant 0.5, FP32" % n, "cells": cells,
-                   "graph": not args.no_graph},
+                   "graph": not args.no_graph, "fused_e_ghosts": fused},
         "hbm_pct": round(100 * achieved / HBM_PEAK_GBPS, 1),
         "fp64": fp64,
         "batched": batched,

@@ -93,6 +93,7 @@ _SIGNATURES = {
     "gmes_sim_set_graph": (C.c_int, [_P, C.c_int32]),
     "gmes_sim_set_tile": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "gmes_sim_autotune": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "gmes_sim_set_ghost_fusion": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
     "gmes_sim_profile": (C.c_int, [_P, C.c_int32]),
     "gmes_sim_read_profile": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "gmes_batch_create": (C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(_P)]),
@@ -248,6 +249,13 @@ class Library(object):
         bx, by = C.c_int32(), C.c_int32()
         self.call("gmes_sim_autotune", h, C.byref(bx), C.byref(by))
         return bx.value, by.value
+
+    def set_ghost_fusion(self, h, enable):
+        """Allow (1) or forbid (0) the fused E ghost planes; returns
+        whether the fusion is in effect."""
+        fused = C.c_int32()
+        self.call("gmes_sim_set_ghost_fusion", h, int(enable), C.byref(fused))
+        return bool(fused.value)
 
     def profile(self, h, enable):
         self.call("gmes_sim_profile", h, int(enable))

@@ -15,6 +15,7 @@
 #include <map>
 #include <mutex>
 #include <numeric>
+#include <shared_mutex>
 #include <string>
 #include <vector>
 
@@ -42,6 +43,28 @@ int set_error(int code, const char* fmt, ...) {
                        #expr, hipGetErrorString(e_));                                   \
   } while (0)
 
+// Calls that wait for the whole device (hipFree, hipDeviceSynchronize,
+// hipStreamDestroy, copies from pageable memory) invalidate a stream
+// capture running on another thread: a served request is built on a worker
+// thread while the main thread captures the batch step.  Such calls hold
+// this lock shared; a capture holds it exclusively.
+std::shared_mutex g_capture_mu;
+
+hipError_t dev_release(void* p) {
+  std::shared_lock<std::shared_mutex> l(g_capture_mu);
+  return hipFree(p);
+}
+
+hipError_t device_drain() {
+  std::shared_lock<std::shared_mutex> l(g_capture_mu);
+  return hipDeviceSynchronize();
+}
+
+hipError_t stream_release(hipStream_t st) {
+  std::shared_lock<std::shared_mutex> l(g_capture_mu);
+  return hipStreamDestroy(st);
+}
+
 const int kAx1[6] = {1, 2, 0, 1, 2, 0};
 const int kAx2[6] = {2, 0, 1, 2, 0, 1};
 const int kGhostOrder[6][2] = {{1, 2}, {2, 0}, {0, 1}, {1, 2}, {2, 0}, {0, 1}};
@@ -67,7 +90,7 @@ int dev_alloc(gmes_sim* s, void** p, size_t bytes) {
 
 void dev_free(gmes_sim* s, void* p, size_t bytes) {
   if (p) {
-    (void)hipFree(p);
+    (void)dev_release(p);
     s->bytes -= (int64_t)bytes;
   }
 }
@@ -95,9 +118,9 @@ void free_ade(gmes_sim* s) {
   for (int c = 0; c < 6; ++c)
     for (int k = 0; k < GMES_NCLS; ++k) {
       AdeList& L = s->ade[c][k];
-      if (L.cell) (void)hipFree(L.cell);
-      if (L.rec) (void)hipFree(L.rec);
-      if (L.state) (void)hipFree(L.state);
+      if (L.cell) (void)dev_release(L.cell);
+      if (L.rec) (void)dev_release(L.rec);
+      if (L.state) (void)dev_release(L.state);
       L = AdeList();
     }
 }
@@ -288,6 +311,13 @@ bool main_args(const gmes_sim* s, bool electric, MainArgs<T>& m, int ext[3]) {
   m.inv_d[0] = (R)(1.0 / s->g.dx); m.inv_d[1] = (R)(1.0 / s->g.dy); m.inv_d[2] = (R)(1.0 / s->g.dz);
   m.nrec = (int)s->recs.size();
   m.n[0] = s->g.nx; m.n[1] = s->g.ny; m.n[2] = s->g.nz;
+  m.fuse = electric && s->fuse_e;
+  for (int c = 0; c < 3; ++c)
+    for (int t = 0; t < 2; ++t) {
+      const int a = kGhostOrder[c][t];
+      m.gax[c][t] = a;
+      m.gph[c][t] = cplx<R>{(R)s->g.phase[c][a][0], (R)s->g.phase[c][a][1]};
+    }
   return true;
 }
 
@@ -432,7 +462,7 @@ int upload_dft_jobs_t(gmes_sim* s) {
   // would wait for every kernel on the device)
   const int64_t need = std::max<int64_t>(16, (int64_t)jobs.size());
   if (need > s->dft_jobs_cap) {
-    if (s->d_dft_jobs) HIP_TRY(hipFree(s->d_dft_jobs));
+    if (s->d_dft_jobs) HIP_TRY(dev_release(s->d_dft_jobs));
     s->d_dft_jobs = nullptr;
     HIP_TRY(hipMalloc(&s->d_dft_jobs, (size_t)need * sizeof(DftJob<T>)));
     s->dft_jobs_cap = need;
@@ -476,7 +506,7 @@ void enqueue_step_t(gmes_sim* s, hipStream_t st) {
   enqueue_src<T>(s, true, st);
   enqueue_probe<T>(s, true, st);
   enqueue_aux_steps(s, st);
-  enqueue_ghost<T>(s, true, 0, st);
+  if (!s->fuse_e) enqueue_ghost<T>(s, true, 0, st);
   enqueue_main<T>(s, false, st);
   enqueue_src<T>(s, false, st);
   enqueue_probe<T>(s, false, st);
@@ -574,9 +604,12 @@ void destroy_graph(gmes_sim* s) {
 int build_graph(gmes_sim* s) {
   destroy_graph(s);
   hipGraph_t graph;
-  HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-  enqueue_step(s, s->stream);
-  HIP_TRY(hipStreamEndCapture(s->stream, &graph));
+  {
+    std::unique_lock<std::shared_mutex> l(g_capture_mu);
+    HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(s, s->stream);
+    HIP_TRY(hipStreamEndCapture(s->stream, &graph));
+  }
   hipError_t e = hipGraphInstantiate(&s->graph_exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return set_error(GMES_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
@@ -707,11 +740,39 @@ int autotune_t(gmes_sim* s, int& best_bx, int& best_by) {
   (void)hipEventDestroy(e1);
   for (auto& x : saved) {
     HIP_TRY(hipMemcpy(x.second.first, x.first, x.second.second, hipMemcpyDeviceToDevice));
-    HIP_TRY(hipFree(x.first));
+    HIP_TRY(dev_release(x.first));
   }
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tune_cache[key] = {best_bx, best_by};
   return GMES_OK;
+}
+
+// Whether the 3-D E kernel may write the E ghost planes itself instead of
+// the separate copy that runs later in the step (after the dispersive
+// updates, the E sources and the E probes).  The results are the same when
+// none of those touches a plane involved in the copy: no dispersive cell
+// on a source plane (plane 0 of a transverse axis), no E source operation
+// on a source or ghost plane, no E probe on a ghost plane (it would read
+// the ghost before the copy).  gmes_sim_set_ghost_fusion(sim, 0) turns it off.
+bool on_ghost_copy_plane(const gmes_sim* s, int c, int64_t x, bool source, bool ghost) {
+  const int64_t sy = s->shape[c][1], sz = s->shape[c][2];
+  const int64_t ix[3] = {x / (sy * sz), (x / sz) % sy, x % sz};
+  for (int t = 0; t < 2; ++t) {
+    const int a = kGhostOrder[c][t];
+    if ((source && ix[a] == 0) || (ghost && ix[a] == s->shape[c][a] - 1)) return true;
+  }
+  return false;
+}
+
+void decide_fusion(gmes_sim* s) {
+  const bool prev = s->fuse_e;
+  bool f = s->allow_fuse && is3d(s) && s->committed && !s->ade_on_ghost_source;
+  for (size_t n = 0; f && n < s->ops.size(); ++n)
+    if (s->ops[n].comp < 3 && on_ghost_copy_plane(s, s->ops[n].comp, s->ops[n].cell, true, true)) f = false;
+  for (size_t n = 0; f && n < s->probes.size(); ++n)
+    if (s->probes[n].comp < 3 && on_ghost_copy_plane(s, s->probes[n].comp, s->probes[n].cell, false, true)) f = false;
+  s->fuse_e = f;
+  if (f != prev) destroy_graph(s);
 }
 
 int ctl_write(gmes_sim* s) {
@@ -806,19 +867,19 @@ int gmes_sim_destroy(gmes_sim* s) {
   destroy_graph(s);
   collect_timing(s);
   for (int c = 0; c < 6; ++c) {
-    if (s->f[c]) (void)hipFree(s->f[c]);
-    if (s->ids[c]) (void)hipFree(s->ids[c]);
-    if (s->pml_state[c]) (void)hipFree(s->pml_state[c]);
+    if (s->f[c]) (void)dev_release(s->f[c]);
+    if (s->ids[c]) (void)dev_release(s->ids[c]);
+    if (s->pml_state[c]) (void)dev_release(s->pml_state[c]);
   }
   free_ade(s);
   for (auto& d : s->dfts)
-    if (d.acc) (void)hipFree(d.acc);
+    if (d.acc) (void)dev_release(d.acc);
   void* bufs[] = {s->d_recs, s->d_coef, s->d_prof, s->d_prof_off, s->d_ops, s->d_groups, s->d_waves,
                   s->d_aux_ptrs, s->d_probe_cells, s->d_probe_comps, s->d_probe_buf, s->d_clock,
                   s->d_dft_w, s->d_dft_jobs};
   for (void* b : bufs)
-    if (b) (void)hipFree(b);
-  if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (b) (void)dev_release(b);
+  if (s->stream) (void)stream_release(s->stream);
   delete s;
   return GMES_OK;
 }
@@ -904,6 +965,7 @@ int gmes_sim_commit(gmes_sim* s) {
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(sync_exec(s));
   free_ade(s);
+  s->ade_on_ghost_source = false;
   for (int c = 0; c < 6; ++c) {
     for (int k = 0; k < GMES_NCLS; ++k) s->has_cls[c][k] = false;
     if (!active(s, c)) continue;
@@ -962,6 +1024,7 @@ int gmes_sim_commit(gmes_sim* s) {
         if (r.cls >= GMES_CLS_DRUDE) {
           cells[r.cls].push_back((int32_t)x);
           recs[r.cls].push_back(ids[x]);
+          if (c < 3 && on_ghost_copy_plane(s, c, x, true, false)) s->ade_on_ghost_source = true;
         }
       }
     // full-grid CPML (2 words) / UPML (1 word) state
@@ -988,14 +1051,15 @@ int gmes_sim_commit(gmes_sim* s) {
       HIP_TRY(hipMemset(L.state, 0, sb));
     }
   }
-  if (s->d_recs) { (void)hipFree(s->d_recs); s->d_recs = nullptr; }
-  if (s->d_coef) { (void)hipFree(s->d_coef); s->d_coef = nullptr; }
-  if (s->d_prof) { (void)hipFree(s->d_prof); s->d_prof = nullptr; }
-  if (s->d_prof_off) { (void)hipFree(s->d_prof_off); s->d_prof_off = nullptr; }
+  if (s->d_recs) { (void)dev_release(s->d_recs); s->d_recs = nullptr; }
+  if (s->d_coef) { (void)dev_release(s->d_coef); s->d_coef = nullptr; }
+  if (s->d_prof) { (void)dev_release(s->d_prof); s->d_prof = nullptr; }
+  if (s->d_prof_off) { (void)dev_release(s->d_prof_off); s->d_prof_off = nullptr; }
   int rc = s->fp64 ? upload_tables<double>(s) : upload_tables<float>(s);
   if (rc) return rc;
   s->committed = true;
   destroy_graph(s);
+  decide_fusion(s);
   return GMES_OK;
 }
 
@@ -1026,6 +1090,7 @@ int gmes_sim_set_sources(gmes_sim* s, const gmes_src_op* ops, int64_t nops, int3
   s->nwaves = nwaves;
   int rc = s->fp64 ? upload_sources<double>(s) : upload_sources<float>(s);
   destroy_graph(s);
+  decide_fusion(s);
   return rc;
 }
 
@@ -1077,9 +1142,9 @@ int gmes_sim_add_probe(gmes_sim* s, int32_t c, int64_t cell, int32_t* id) {
   std::vector<int64_t> cells;
   std::vector<int32_t> comps;
   for (auto& p : s->probes) { cells.push_back(dev_flat(s, p.comp, p.cell)); comps.push_back(p.comp); }
-  if (s->d_probe_cells) (void)hipFree(s->d_probe_cells);
-  if (s->d_probe_comps) (void)hipFree(s->d_probe_comps);
-  if (s->d_probe_buf) (void)hipFree(s->d_probe_buf);
+  if (s->d_probe_cells) (void)dev_release(s->d_probe_cells);
+  if (s->d_probe_comps) (void)dev_release(s->d_probe_comps);
+  if (s->d_probe_buf) (void)dev_release(s->d_probe_buf);
   s->d_probe_buf = nullptr;
   s->probe_cap = 0;
   HIP_TRY(hipMalloc((void**)&s->d_probe_cells, cells.size() * sizeof(int64_t)));
@@ -1088,6 +1153,7 @@ int gmes_sim_add_probe(gmes_sim* s, int32_t c, int64_t cell, int32_t* id) {
   HIP_TRY(hipMemcpy(s->d_probe_comps, comps.data(), comps.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   *id = (int32_t)s->probes.size() - 1;
   destroy_graph(s);
+  decide_fusion(s);
   return GMES_OK;
 }
 
@@ -1160,7 +1226,7 @@ int gmes_sim_set_dft_weights(gmes_sim* s, int64_t step0, int64_t nsteps, const d
   HIP_TRY(sync_exec(s));
   const int64_t need = std::max<int64_t>(1, nsteps * s->dft_nfreq * 4);
   if (need > s->dft_w_cap) {
-    if (s->d_dft_w) HIP_TRY(hipFree(s->d_dft_w));
+    if (s->d_dft_w) HIP_TRY(dev_release(s->d_dft_w));
     s->d_dft_w = nullptr;
     HIP_TRY(hipMalloc((void**)&s->d_dft_w, (size_t)need * sizeof(double)));
     s->dft_w_cap = need;
@@ -1195,9 +1261,12 @@ int gmes_sim_write_field(gmes_sim* s, int32_t c, const void* host, int64_t bytes
     HIP_TRY(sync_exec(s));
     return GMES_OK;
   }
-  HIP_TRY(hipMemcpy2DAsync(static_cast<char*>(s->f[c]) + (size_t)(kPad - s->koff[c]) * s->esize,
-                           (size_t)s->pitch[c] * s->esize, host, row, row,
-                           (size_t)(s->shape[c][0] * s->shape[c][1]), hipMemcpyHostToDevice, s->stream));
+  {
+    std::shared_lock<std::shared_mutex> l(g_capture_mu);  // pageable 2-D copy: waits for the device
+    HIP_TRY(hipMemcpy2DAsync(static_cast<char*>(s->f[c]) + (size_t)(kPad - s->koff[c]) * s->esize,
+                             (size_t)s->pitch[c] * s->esize, host, row, row,
+                             (size_t)(s->shape[c][0] * s->shape[c][1]), hipMemcpyHostToDevice, s->stream));
+  }
   HIP_TRY(sync_exec(s));
   return GMES_OK;
 }
@@ -1213,9 +1282,13 @@ int gmes_sim_read_field(gmes_sim* s, int32_t c, void* host, int64_t bytes) {
     HIP_TRY(sync_exec(s));
     return GMES_OK;
   }
-  HIP_TRY(hipMemcpy2DAsync(host, row, static_cast<const char*>(s->f[c]) + (size_t)(kPad - s->koff[c]) * s->esize,
-                           (size_t)s->pitch[c] * s->esize, row, (size_t)(s->shape[c][0] * s->shape[c][1]),
-                           hipMemcpyDeviceToHost, s->stream));
+  {
+    std::shared_lock<std::shared_mutex> l(g_capture_mu);
+    HIP_TRY(hipMemcpy2DAsync(host, row,
+                             static_cast<const char*>(s->f[c]) + (size_t)(kPad - s->koff[c]) * s->esize,
+                             (size_t)s->pitch[c] * s->esize, row, (size_t)(s->shape[c][0] * s->shape[c][1]),
+                             hipMemcpyDeviceToHost, s->stream));
+  }
   HIP_TRY(sync_exec(s));
   return GMES_OK;
 }
@@ -1291,6 +1364,15 @@ int gmes_sim_get_step(const gmes_sim* s, int64_t* step) {
 int gmes_sim_set_graph(gmes_sim* s, int32_t enable) {
   s->use_graph = enable != 0;
   destroy_graph(s);
+  return GMES_OK;
+}
+
+int gmes_sim_set_ghost_fusion(gmes_sim* s, int32_t enable, int32_t* fused) {
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(sync_exec(s));
+  s->allow_fuse = enable != 0;
+  decide_fusion(s);
+  if (fused) *fused = s->fuse_e ? 1 : 0;
   return GMES_OK;
 }
 
@@ -1421,6 +1503,7 @@ void build_jobs(gmes_batch* b) {
         nbs += (S.a.ngroups + 255) / 256;
         js.push_back(S);
       }
+      if (electric && s->fuse_e) continue;  // E ghosts written by the 3-D E kernel
       GhostJob<T> G{};
       int64_t total = 0;
       G.g = ghost_args<T>(s, electric, 0, total);
@@ -1538,7 +1621,7 @@ int gmes_batch_create(gmes_sim* const* sims, int32_t n, gmes_batch** out) {
   }
   // setup work of the members issued on the null stream (zero fills, table
   // uploads) is not ordered before the non-blocking batch stream: drain it
-  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(device_drain());
   *out = b;
   return GMES_OK;
 }
@@ -1549,8 +1632,8 @@ int gmes_batch_destroy(gmes_batch* b) {
   (void)hipStreamSynchronize(b->stream);
   destroy_batch_graph(b);
   for (gmes_sim* s : b->sims) s->batch_stream = nullptr;
-  if (b->d_jobs) (void)hipFree(b->d_jobs);
-  (void)hipStreamDestroy(b->stream);
+  if (b->d_jobs) (void)dev_release(b->d_jobs);
+  (void)stream_release(b->stream);
   delete b;
   return GMES_OK;
 }
@@ -1582,7 +1665,7 @@ int gmes_batch_step(gmes_batch* b, int64_t nsteps) {
   }
   if (b->h_jobs.size() > b->jobs_cap) {
     destroy_batch_graph(b);
-    if (b->d_jobs) HIP_TRY(hipFree(b->d_jobs));
+    if (b->d_jobs) HIP_TRY(dev_release(b->d_jobs));
     b->d_jobs = nullptr;
     b->jobs_cap = 0;
     HIP_TRY(hipMalloc(&b->d_jobs, b->h_jobs.size()));
@@ -1617,9 +1700,12 @@ int gmes_batch_step(gmes_batch* b, int64_t nsteps) {
   if (b->use_graph) {
     if (!b->graph_exec) {
       hipGraph_t graph;
-      HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-      enqueue_batch_step(b, b->stream);
-      HIP_TRY(hipStreamEndCapture(b->stream, &graph));
+      {
+        std::unique_lock<std::shared_mutex> l(g_capture_mu);
+        HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_batch_step(b, b->stream);
+        HIP_TRY(hipStreamEndCapture(b->stream, &graph));
+      }
       hipError_t e = hipGraphInstantiate(&b->graph_exec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
       if (e != hipSuccess) return set_error(GMES_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));

@@ -111,6 +111,10 @@ struct gmes_sim {
   int32_t naux = 0;
   gmes_sim* parent = nullptr;  // the sim whose stream advances this aux sim
   hipStream_t batch_stream = nullptr;  // stream of the batch this sim belongs to
+  // E ghost planes written by the 3-D E kernel (no standalone E ghost copy);
+  // ade_on_ghost_source: a dispersive E cell lies on plane 0 of a
+  // transverse axis (set by commit), which rules that out
+  bool fuse_e = false, ade_on_ghost_source = false, allow_fuse = true;
   uint64_t gen = 0;            // bumped whenever captured launch arguments go stale
   void* d_aux_ptrs = nullptr;  // [8][6] device pointers of aux fields
   // probes

@@ -57,6 +57,10 @@ struct MainArgs {
   R inv_d[3];
   int nrec;
   int n[3];  // nx, ny, nz
+  // 1: the 3-D E kernel writes the E ghost planes itself (fused_ghost)
+  int fuse;
+  int gax[3][2];      // transverse axes a1, a2 of Ex, Ey, Ez
+  cplx<R> gph[3][2];  // ghost phase along a1, a2
 };
 
 template <class T>

@@ -198,16 +198,56 @@ __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typ
 // :96,198), Dielectric, CPML, UPML.  ADE classes are left to their kernels.
 // f is the cell's old value, loaded by the caller.
 template <int C, class T>
-__device__ __forceinline__ void store_cell(const MainArgs<T>& m, const DevRec<typename real_of<T>::type>& rec,
-                                           T* __restrict__ F, T* __restrict__ st, int flat, int i, int j, int k,
-                                           T f, T D1, T D2) {
+__device__ __forceinline__ T store_cell(const MainArgs<T>& m, const DevRec<typename real_of<T>::type>& rec,
+                                        T* __restrict__ F, T* __restrict__ st, int flat, int i, int j, int k,
+                                        T f, T D1, T D2) {
   const int cls = rec.cls;
-  if (cls == GMES_CLS_DUMMY || cls >= GMES_CLS_DRUDE) return;
+  if (cls == GMES_CLS_DUMMY || cls >= GMES_CLS_DRUDE) return f;
   if (cls == GMES_CLS_CONST) {
-    st_nt(F + flat, make_val<T>(rec.val_re, rec.val_im));
-    return;
+    const T v = make_val<T>(rec.val_re, rec.val_im);
+    st_nt(F + flat, v);
+    return v;
   }
-  st_nt(F + flat, update_value<C>(m, rec, st, flat, i, j, k, f, D1, D2));
+  const T v = update_value<C>(m, rec, st, flat, i, j, k, f, D1, D2);
+  st_nt(F + flat, v);
+  return v;
+}
+
+// Periodic / Bloch ghost planes of E written by the thread that owns the
+// source cell (the copy k_ghost<E> makes, fused into the E update when
+// nothing else writes or reads the planes involved before that copy would
+// run).  x is the cell's new value; ix its storage index.  Same phase
+// products and corner order as k_ghost: ph1 * x on the a1 ghost plane,
+// ph2 * x on the a2 plane, ph2 * (ph1 * x) on the corner line.
+template <class T>
+__device__ __forceinline__ T ghost_phase(const cplx<typename real_of<T>::type>& ph, T x) {
+  if constexpr (is_cplx<T>::value) return cmul(ph, x);
+  else return x;
+}
+
+template <class T>
+__device__ __forceinline__ void fused_ghost(const MainArgs<T>& m, int C, T* __restrict__ F, const int ix[3], T x) {
+  const int a1 = m.gax[C][0], a2 = m.gax[C][1];
+  const int* sh = m.fa.s[C];
+  const int dst1 = sh[a1] - 1, dst2 = sh[a2] - 1;
+  const bool on1 = ix[a1] == 0, on2 = ix[a2] == 0;
+  if (!on1 && !on2) return;
+  int g[3] = {ix[0], ix[1], ix[2]};
+  T x1 = x;
+  if (on1) {
+    x1 = ghost_phase<T>(m.gph[C][0], x);
+    g[a1] = dst1;
+    F[didx(m.fa, C, g[0], g[1], g[2])] = x1;
+    g[a1] = 0;
+  }
+  if (on2) {
+    g[a2] = dst2;
+    F[didx(m.fa, C, g[0], g[1], g[2])] = ghost_phase<T>(m.gph[C][1], x);
+    if (on1) {
+      g[a1] = dst1;
+      F[didx(m.fa, C, g[0], g[1], g[2])] = ghost_phase<T>(m.gph[C][1], x1);
+    }
+  }
 }
 
 template <int C, class T>
@@ -216,8 +256,8 @@ __device__ __forceinline__ void update_cell(const MainArgs<T>& m, const DevRec<t
   const int cls = rec.cls;
   if (cls == GMES_CLS_DUMMY || cls >= GMES_CLS_DRUDE) return;
   T* F = m.fa.f[C];
-  store_cell<C>(m, rec, F, static_cast<T*>(m.pml[C]), flat, i, j, k, cls == GMES_CLS_CONST ? vzero<T>() : F[flat],
-                D1, D2);
+  (void)store_cell<C>(m, rec, F, static_cast<T*>(m.pml[C]), flat, i, j, k,
+                      cls == GMES_CLS_CONST ? vzero<T>() : F[flat], D1, D2);
 }
 
 template <int C, class T>
@@ -308,12 +348,18 @@ __device__ __forceinline__ void update_e(const MainArgs<T>& m, const DevRec<type
     const int i = i0 + n;
     const int fx = B[0] + i * P[0] + j * W[0] + k, fy = B[1] + i * P[1] + j * W[1] + k;
     const int fz = B[2] + i * P[2] + j * W[2] + k;
-    store_cell<0>(m, recs[q.r[0][n]], ex, sx, fx, i, j, k, q.o[0][n], vsub(q.c[5][n], q.c[6][n]),
-                  vsub(q.c[3][n], q.c[4][n]));
-    store_cell<1>(m, recs[q.r[1][n]], ey, sy, fy, i, j, k, q.o[1][n], vsub(q.c[0][n], q.c[2][n]),
-                  vsub(q.c[5][n], hz_prev));
-    store_cell<2>(m, recs[q.r[2][n]], ez, sz, fz, i, j, k, q.o[2][n], vsub(q.c[3][n], hy_prev),
-                  vsub(q.c[0][n], q.c[1][n]));
+    const T vx = store_cell<0>(m, recs[q.r[0][n]], ex, sx, fx, i, j, k, q.o[0][n], vsub(q.c[5][n], q.c[6][n]),
+                               vsub(q.c[3][n], q.c[4][n]));
+    const T vy = store_cell<1>(m, recs[q.r[1][n]], ey, sy, fy, i, j, k, q.o[1][n], vsub(q.c[0][n], q.c[2][n]),
+                               vsub(q.c[5][n], hz_prev));
+    const T vz = store_cell<2>(m, recs[q.r[2][n]], ez, sz, fz, i, j, k, q.o[2][n], vsub(q.c[3][n], hy_prev),
+                               vsub(q.c[0][n], q.c[1][n]));
+    if (m.fuse && (i == 0 || j == 0 || k == 0)) {
+      const int ix[3] = {i, j, k};
+      fused_ghost<T>(m, 0, ex, ix, vx);
+      fused_ghost<T>(m, 1, ey, ix, vy);
+      fused_ghost<T>(m, 2, ez, ix, vz);
+    }
     hz_prev = q.c[5][n];
     hy_prev = q.c[3][n];
   }
@@ -360,12 +406,12 @@ __device__ __forceinline__ void update_h(const MainArgs<T>& m, const DevRec<type
     const int fx = B[3] + i * P[3] + (j + 1) * W[3] + k + 1;
     const int fy = B[4] + (i + 1) * P[4] + j * W[4] + k + 1;
     const int fz = B[5] + (i + 1) * P[5] + (j + 1) * W[5] + k;
-    store_cell<3>(m, recs[q.r[0][n]], hx, sx, fx, i, j + 1, k + 1, q.o[0][n], vsub(q.c[5][n], ez_cur),
-                  vsub(q.c[3][n], ey_cur));
-    store_cell<4>(m, recs[q.r[1][n]], hy, sy, fy, i + 1, j, k + 1, q.o[1][n], vsub(q.c[1][n], q.c[0][n]),
-                  vsub(q.c[6][n], ez_cur));
-    store_cell<5>(m, recs[q.r[2][n]], hz, sz, fz, i + 1, j + 1, k, q.o[2][n], vsub(q.c[4][n], ey_cur),
-                  vsub(q.c[2][n], q.c[0][n]));
+    const T vx = store_cell<3>(m, recs[q.r[0][n]], hx, sx, fx, i, j + 1, k + 1, q.o[0][n], vsub(q.c[5][n], ez_cur),
+                               vsub(q.c[3][n], ey_cur));
+    const T vy = store_cell<4>(m, recs[q.r[1][n]], hy, sy, fy, i + 1, j, k + 1, q.o[1][n],
+                               vsub(q.c[1][n], q.c[0][n]), vsub(q.c[6][n], ez_cur));
+    const T vz = store_cell<5>(m, recs[q.r[2][n]], hz, sz, fz, i + 1, j + 1, k, q.o[2][n], vsub(q.c[4][n], ey_cur),
+                               vsub(q.c[2][n], q.c[0][n]));
     ey_cur = q.c[4][n];
     ez_cur = q.c[6][n];
   }
@@ -866,6 +912,7 @@ __device__ __forceinline__ void ghost_point(const GhostArgs<T>& m, int64_t e) {
   int t = 0;
   while (t < m.ncomp && e >= m.count[t]) { e -= m.count[t]; ++t; }
   if (t >= m.ncomp) return;
+  int r = (int)e;  // a point of one component: 32-bit index arithmetic
   const int c = m.comp[t];
   const int a1 = m.ax[t][0], a2 = m.ax[t][1], a0 = m.ax[t][2];
   const int* s = m.fa.s[c];
@@ -874,11 +921,14 @@ __device__ __forceinline__ void ghost_point(const GhostArgs<T>& m, int64_t e) {
   const int dst2 = electric ? s[a2] - 1 : 0, src2 = electric ? 0 : s[a2] - 1;
   int ix[3];
   T v;
-  const int64_t plane1 = (int64_t)s[a0] * s[a2];
+  const int plane1 = s[a0] * s[a2];
   T* f = m.fa.f[c];
-  if (e < plane1) {
-    ix[a0] = (int)(e / s[a2]);
-    ix[a2] = (int)(e % s[a2]);
+  // consecutive points run along the free axis of higher index (the one
+  // that is contiguous, or closer to contiguous, in memory)
+  if (r < plane1) {
+    const int fa = a0 > a2 ? a0 : a2, sa = a0 > a2 ? a2 : a0;
+    ix[sa] = r / s[fa];
+    ix[fa] = r - ix[sa] * s[fa];
     ix[a1] = dst1;
     int sx[3] = {ix[0], ix[1], ix[2]};
     sx[a1] = src1;
@@ -889,10 +939,16 @@ __device__ __forceinline__ void ghost_point(const GhostArgs<T>& m, int64_t e) {
       if (ix[a2] == dst2) v = cmul(m.ph[t][1], v);
     }
   } else {
-    e -= plane1;
-    const int w1 = s[a1] - 1;
-    ix[a0] = (int)(e / w1);
-    const int w = (int)(e % w1);
+    r -= plane1;
+    const int w1 = s[a1] - 1;  // a1 coordinates off the first ghost plane
+    int w;
+    if (a1 > a0) {
+      ix[a0] = r / w1;
+      w = r - ix[a0] * w1;
+    } else {
+      w = r / s[a0];
+      ix[a0] = r - w * s[a0];
+    }
     ix[a1] = electric ? w : w + 1;
     ix[a2] = dst2;
     int sx[3] = {ix[0], ix[1], ix[2]};

@@ -232,6 +232,14 @@ int gmes_sim_set_tile(gmes_sim* sim, int32_t bx, int32_t by);
  * the chosen tile.  No reference counterpart: the reference's update loops
  * have no launch shape (src/cpp23_support.hh:147-197). */
 int gmes_sim_autotune(gmes_sim* sim, int32_t* bx, int32_t* by);
+/* Fused E ghost planes.  enable = 1 (the default) lets the 3-D E kernel
+ * write the periodic / Bloch E ghost planes itself, in place of the copy
+ * that follows the E sources (gmes/fdtd.py:955-1097), whenever that gives
+ * the same fields: no dispersive E cell on a source plane, no E source
+ * operation on a source or ghost plane, no E probe on a ghost plane.
+ * enable = 0 keeps the separate copy.  *fused (may be NULL) reports
+ * whether the fusion is in effect now.  No reference counterpart. */
+int gmes_sim_set_ghost_fusion(gmes_sim* sim, int32_t enable, int32_t* fused);
 /* Kernel timing on the sim's stream: enable accumulates HIP-event time per
  * kernel family; read returns ms and launch counts for
  * [E main, H main, E ADE, sources, ghost, aux].  */

@@ -542,6 +542,13 @@ int gmes_sim_autotune(gmes_sim *s, int32_t *bx, int32_t *by) {
   if (by) *by = 0;
   return GMES_OK;
 }
+/* the oracle always makes the separate ghost copy */
+int gmes_sim_set_ghost_fusion(gmes_sim *s, int32_t enable, int32_t *fused) {
+  (void)s;
+  (void)enable;
+  if (fused) *fused = 0;
+  return GMES_OK;
+}
 int gmes_sim_profile(gmes_sim *s, int32_t enable) { (void)s; (void)enable; return GMES_OK; }
 int gmes_sim_read_profile(gmes_sim *s, double ms[6], int64_t launches[6]) {
   (void)s;

@@ -237,3 +237,26 @@ def test_dispersive_bloch_cell_c64(engine_lib):
     a._advance(80)
     b._advance(80)
     assert rel_l2(fields(a), fields(b)) <= 1e-5
+
+
+@pytest.mark.parametrize("builder,kw", [(bloch3d, {}), (spheres3d, dict(n=16, nsph=3, pml=3)),
+                                        (tfsf3d, dict(bloch=(0.3, 0, 0.1)))],
+                         ids=["bloch3d", "spheres3d", "tfsf3d_bloch"])
+def test_fused_e_ghosts_match_separate_copy(engine_lib, builder, kw):
+    """E ghost planes written by the 3-D E kernel give the same fields, bit
+    for bit, as the separate ghost copy after the E sources."""
+    a, b = builder(engine_lib, **kw), builder(engine_lib, **kw)
+    assert engine_lib.set_ghost_fusion(a._sim, 1)
+    assert not engine_lib.set_ghost_fusion(b._sim, 0)
+    for _ in range(3):
+        a._advance(11)
+        b._advance(11)
+        fa, fb = fields(a), fields(b)
+        for c in fa:
+            np.testing.assert_array_equal(fa[c], fb[c])
+
+
+def test_ghost_fusion_off_for_dispersive_source_planes(engine_lib):
+    """The dispersive Bloch cell's DCP slab covers the x and y source planes:
+    its E ghosts keep the separate copy."""
+    assert not engine_lib.set_ghost_fusion(_c3(engine_lib)._sim, 1)
