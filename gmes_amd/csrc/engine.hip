@@ -43,27 +43,41 @@ int set_error(int code, const char* fmt, ...) {
                        #expr, hipGetErrorString(e_));                                   \
   } while (0)
 
-// Calls that wait for the whole device (hipFree, hipDeviceSynchronize,
-// hipStreamDestroy, copies from pageable memory) invalidate a stream
-// capture running on another thread: a served request is built on a worker
-// thread while the main thread captures the batch step.  Such calls hold
-// this lock shared; a capture holds it exclusively.
+// Engine calls made from several threads (a served request is built on a
+// worker thread while the main thread captures the batch step) must not
+// run inside a stream capture: a synchronous copy, memset or free issued
+// meanwhile can invalidate it.  Every entry point holds this lock shared
+// (once per thread, however deeply entry points nest); a capture holds it
+// exclusively, so it runs while no other thread is inside the engine.
 std::shared_mutex g_capture_mu;
+thread_local int tl_api_depth = 0;
 
-hipError_t dev_release(void* p) {
-  std::shared_lock<std::shared_mutex> l(g_capture_mu);
-  return hipFree(p);
-}
+struct ApiGuard {
+  ApiGuard() {
+    if (tl_api_depth++ == 0) g_capture_mu.lock_shared();
+  }
+  ~ApiGuard() {
+    if (--tl_api_depth == 0) g_capture_mu.unlock_shared();
+  }
+};
 
-hipError_t device_drain() {
-  std::shared_lock<std::shared_mutex> l(g_capture_mu);
-  return hipDeviceSynchronize();
-}
+// Exclusive for the duration of a capture; the calling thread's own shared
+// hold is released first and taken back afterwards.
+struct CaptureGuard {
+  bool held;
+  CaptureGuard() : held(tl_api_depth > 0) {
+    if (held) g_capture_mu.unlock_shared();
+    g_capture_mu.lock();
+  }
+  ~CaptureGuard() {
+    g_capture_mu.unlock();
+    if (held) g_capture_mu.lock_shared();
+  }
+};
 
-hipError_t stream_release(hipStream_t st) {
-  std::shared_lock<std::shared_mutex> l(g_capture_mu);
-  return hipStreamDestroy(st);
-}
+hipError_t dev_release(void* p) { return hipFree(p); }
+hipError_t device_drain() { return hipDeviceSynchronize(); }
+hipError_t stream_release(hipStream_t st) { return hipStreamDestroy(st); }
 
 const int kAx1[6] = {1, 2, 0, 1, 2, 0};
 const int kAx2[6] = {2, 0, 1, 2, 0, 1};
@@ -605,7 +619,7 @@ int build_graph(gmes_sim* s) {
   destroy_graph(s);
   hipGraph_t graph;
   {
-    std::unique_lock<std::shared_mutex> l(g_capture_mu);
+    CaptureGuard capture;
     HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     enqueue_step(s, s->stream);
     HIP_TRY(hipStreamEndCapture(s->stream, &graph));
@@ -799,6 +813,7 @@ const char* gmes_last_error(void) { return g_err; }
 int gmes_abi_version(void) { return GMES_ABI_VERSION; }
 
 int gmes_device_count(int32_t* count) {
+  ApiGuard api;
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
   *count = n;
@@ -806,6 +821,7 @@ int gmes_device_count(int32_t* count) {
 }
 
 int gmes_sim_create(const gmes_grid_desc* desc, gmes_sim** out) {
+  ApiGuard api;
   if (!desc || !out) return set_error(GMES_EINVAL, "null argument");
   if (desc->dtype < GMES_F32 || desc->dtype > GMES_C128) return set_error(GMES_ETYPE, "bad dtype");
   if (desc->nx < 1 || desc->ny < 1 || desc->nz < 1) return set_error(GMES_EINVAL, "grid sizes must be positive");
@@ -857,6 +873,7 @@ int gmes_sim_create(const gmes_grid_desc* desc, gmes_sim** out) {
 }
 
 int gmes_sim_destroy(gmes_sim* s) {
+  ApiGuard api;
   if (!s) return GMES_OK;
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
@@ -885,18 +902,21 @@ int gmes_sim_destroy(gmes_sim* s) {
 }
 
 int gmes_sim_shape(const gmes_sim* s, int32_t c, int64_t shape[3]) {
+  ApiGuard api;
   if (c < 0 || c > 5) return set_error(GMES_EINVAL, "bad component");
   for (int a = 0; a < 3; ++a) shape[a] = s->shape[c][a];
   return GMES_OK;
 }
 
 int gmes_sim_footprint(const gmes_sim* s, int64_t* bytes) {
+  ApiGuard api;
   *bytes = s->bytes;
   return GMES_OK;
 }
 
 int gmes_sim_set_materials(gmes_sim* s, const gmes_mat_rec* recs, int32_t nrec, const double* coef,
                            int64_t ncoef) {
+  ApiGuard api;
   if (nrec < 1 || nrec > 65536) return set_error(GMES_EINVAL, "record count must be in [1, 65536]");
   if (ncoef < 0) return set_error(GMES_EINVAL, "negative coefficient count");
   s->recs.assign(recs, recs + nrec);
@@ -928,6 +948,7 @@ int gmes_sim_set_materials(gmes_sim* s, const gmes_mat_rec* recs, int32_t nrec, 
 }
 
 int gmes_sim_set_material_ids(gmes_sim* s, int32_t c, const uint16_t* ids, int64_t count) {
+  ApiGuard api;
   if (c < 0 || c > 5) return set_error(GMES_EINVAL, "bad component");
   if (count != s->ncell[c]) return set_error(GMES_EINVAL, "id count %lld != cells %lld", (long long)count,
                                              (long long)s->ncell[c]);
@@ -950,6 +971,7 @@ int gmes_sim_set_material_ids(gmes_sim* s, int32_t c, const uint16_t* ids, int64
 
 int gmes_sim_set_profile(gmes_sim* s, int32_t p, int32_t c, int32_t axis, const double* vals, int32_t n,
                          int32_t width) {
+  ApiGuard api;
   if (p < 0 || p >= 64 || c < 0 || c > 5 || axis < 0 || axis > 2)
     return set_error(GMES_EINVAL, "bad profile address");
   if (n != s->shape[c][axis] || (width != 3 && width != 6)) return set_error(GMES_EINVAL, "bad profile size");
@@ -961,6 +983,7 @@ int gmes_sim_set_profile(gmes_sim* s, int32_t p, int32_t c, int32_t axis, const 
 }
 
 int gmes_sim_commit(gmes_sim* s) {
+  ApiGuard api;
   if (s->recs.empty()) return set_error(GMES_EINVAL, "materials not set");
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(sync_exec(s));
@@ -1064,6 +1087,7 @@ int gmes_sim_commit(gmes_sim* s) {
 }
 
 int gmes_sim_set_sources(gmes_sim* s, const gmes_src_op* ops, int64_t nops, int32_t nwaves) {
+  ApiGuard api;
   if (nops < 0 || nwaves < 0) return set_error(GMES_EINVAL, "negative count");
   if (nops > (1 << 30)) return set_error(GMES_EINVAL, "too many source operations");
   for (int64_t i = 0; i < nops; ++i) {
@@ -1095,6 +1119,7 @@ int gmes_sim_set_sources(gmes_sim* s, const gmes_src_op* ops, int64_t nops, int3
 }
 
 int gmes_sim_set_waveforms(gmes_sim* s, int64_t step0, int64_t nsteps, const double* values) {
+  ApiGuard api;
   if (nsteps < 0) return set_error(GMES_EINVAL, "negative step count");
   HIP_TRY(hipSetDevice(s->device));
   const int64_t need = std::max<int64_t>(1, (int64_t)s->nwaves * nsteps * 2);
@@ -1115,6 +1140,7 @@ int gmes_sim_set_waveforms(gmes_sim* s, int64_t step0, int64_t nsteps, const dou
 }
 
 int gmes_sim_attach_aux(gmes_sim* s, gmes_sim* aux, int32_t* slot) {
+  ApiGuard api;
   if (!aux || aux == s) return set_error(GMES_EINVAL, "bad auxiliary sim");
   if (s->naux >= 8) return set_error(GMES_EINVAL, "too many auxiliary grids");
   if (aux->g.dtype != s->g.dtype) return set_error(GMES_ETYPE, "aux dtype mismatch");
@@ -1133,6 +1159,7 @@ int gmes_sim_attach_aux(gmes_sim* s, gmes_sim* aux, int32_t* slot) {
 }
 
 int gmes_sim_add_probe(gmes_sim* s, int32_t c, int64_t cell, int32_t* id) {
+  ApiGuard api;
   if (c < 0 || c > 5 || !active(s, c) || cell < 0 || cell >= s->ncell[c])
     return set_error(GMES_ERANGE, "probe index out of bounds");
   if (s->step != s->probe_base) return set_error(GMES_EINVAL, "read pending probe values first");
@@ -1166,6 +1193,7 @@ static std::vector<int64_t>& probe_cursor(gmes_sim* s) {
 }
 
 int gmes_sim_read_probe(gmes_sim* s, int32_t id, double* out, int64_t max, int64_t* n) {
+  ApiGuard api;
   if (id < 0 || id >= (int32_t)s->probes.size()) return set_error(GMES_EINVAL, "bad probe id");
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(sync_exec(s));
@@ -1191,6 +1219,7 @@ int gmes_sim_read_probe(gmes_sim* s, int32_t id, double* out, int64_t max, int64
 
 int gmes_sim_add_dft(gmes_sim* s, int32_t c, const int64_t lo[3], const int64_t hi[3], int32_t nfreq,
                      int32_t decim, int32_t* id) {
+  ApiGuard api;
   if (c < 0 || c > 5 || !active(s, c)) return set_error(GMES_EINVAL, "DFT of an inactive component");
   if (nfreq < 1 || decim < 1) return set_error(GMES_EINVAL, "nfreq and decim must be positive");
   if (!s->dfts.empty() && nfreq != s->dft_nfreq)
@@ -1220,6 +1249,7 @@ int gmes_sim_add_dft(gmes_sim* s, int32_t c, const int64_t lo[3], const int64_t 
 }
 
 int gmes_sim_set_dft_weights(gmes_sim* s, int64_t step0, int64_t nsteps, const double* w) {
+  ApiGuard api;
   if (s->dfts.empty()) return set_error(GMES_EINVAL, "no DFT monitor");
   if (nsteps < 0) return set_error(GMES_EINVAL, "negative step count");
   HIP_TRY(hipSetDevice(s->device));
@@ -1241,6 +1271,7 @@ int gmes_sim_set_dft_weights(gmes_sim* s, int64_t step0, int64_t nsteps, const d
 }
 
 int gmes_sim_read_dft(gmes_sim* s, int32_t id, double* out, int64_t count) {
+  ApiGuard api;
   if (id < 0 || id >= (int32_t)s->dfts.size()) return set_error(GMES_EINVAL, "bad DFT id");
   const auto& d = s->dfts[id];
   if (count != (int64_t)d.nfreq * d.ncells * 2) return set_error(GMES_EINVAL, "size mismatch");
@@ -1251,6 +1282,7 @@ int gmes_sim_read_dft(gmes_sim* s, int32_t id, double* out, int64_t count) {
 }
 
 int gmes_sim_write_field(gmes_sim* s, int32_t c, const void* host, int64_t bytes) {
+  ApiGuard api;
   if (c < 0 || c > 5) return set_error(GMES_EINVAL, "bad component");
   if (bytes != s->ncell[c] * s->esize) return set_error(GMES_EINVAL, "size mismatch");
   HIP_TRY(hipSetDevice(s->device));
@@ -1261,17 +1293,15 @@ int gmes_sim_write_field(gmes_sim* s, int32_t c, const void* host, int64_t bytes
     HIP_TRY(sync_exec(s));
     return GMES_OK;
   }
-  {
-    std::shared_lock<std::shared_mutex> l(g_capture_mu);  // pageable 2-D copy: waits for the device
-    HIP_TRY(hipMemcpy2DAsync(static_cast<char*>(s->f[c]) + (size_t)(kPad - s->koff[c]) * s->esize,
-                             (size_t)s->pitch[c] * s->esize, host, row, row,
-                             (size_t)(s->shape[c][0] * s->shape[c][1]), hipMemcpyHostToDevice, s->stream));
-  }
+  HIP_TRY(hipMemcpy2DAsync(static_cast<char*>(s->f[c]) + (size_t)(kPad - s->koff[c]) * s->esize,
+                           (size_t)s->pitch[c] * s->esize, host, row, row,
+                           (size_t)(s->shape[c][0] * s->shape[c][1]), hipMemcpyHostToDevice, s->stream));
   HIP_TRY(sync_exec(s));
   return GMES_OK;
 }
 
 int gmes_sim_read_field(gmes_sim* s, int32_t c, void* host, int64_t bytes) {
+  ApiGuard api;
   if (c < 0 || c > 5) return set_error(GMES_EINVAL, "bad component");
   if (bytes != s->ncell[c] * s->esize) return set_error(GMES_EINVAL, "size mismatch");
   HIP_TRY(hipSetDevice(s->device));
@@ -1282,13 +1312,9 @@ int gmes_sim_read_field(gmes_sim* s, int32_t c, void* host, int64_t bytes) {
     HIP_TRY(sync_exec(s));
     return GMES_OK;
   }
-  {
-    std::shared_lock<std::shared_mutex> l(g_capture_mu);
-    HIP_TRY(hipMemcpy2DAsync(host, row,
-                             static_cast<const char*>(s->f[c]) + (size_t)(kPad - s->koff[c]) * s->esize,
-                             (size_t)s->pitch[c] * s->esize, row, (size_t)(s->shape[c][0] * s->shape[c][1]),
-                             hipMemcpyDeviceToHost, s->stream));
-  }
+  HIP_TRY(hipMemcpy2DAsync(host, row, static_cast<const char*>(s->f[c]) + (size_t)(kPad - s->koff[c]) * s->esize,
+                           (size_t)s->pitch[c] * s->esize, row, (size_t)(s->shape[c][0] * s->shape[c][1]),
+                           hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(sync_exec(s));
   return GMES_OK;
 }
@@ -1305,6 +1331,7 @@ static void state_blocks(gmes_sim* s, std::vector<std::pair<void*, size_t>>& out
 }
 
 int gmes_sim_state_bytes(const gmes_sim* s, int64_t* bytes) {
+  ApiGuard api;
   std::vector<std::pair<void*, size_t>> b;
   state_blocks(const_cast<gmes_sim*>(s), b);
   int64_t t = 0;
@@ -1314,6 +1341,7 @@ int gmes_sim_state_bytes(const gmes_sim* s, int64_t* bytes) {
 }
 
 int gmes_sim_snapshot(gmes_sim* s, void* host, int64_t bytes) {
+  ApiGuard api;
   std::vector<std::pair<void*, size_t>> b;
   state_blocks(s, b);
   int64_t t = 0;
@@ -1327,6 +1355,7 @@ int gmes_sim_snapshot(gmes_sim* s, void* host, int64_t bytes) {
 }
 
 int gmes_sim_restore(gmes_sim* s, const void* host, int64_t bytes) {
+  ApiGuard api;
   std::vector<std::pair<void*, size_t>> b;
   state_blocks(s, b);
   int64_t t = 0;
@@ -1339,9 +1368,13 @@ int gmes_sim_restore(gmes_sim* s, const void* host, int64_t bytes) {
   return GMES_OK;
 }
 
-int gmes_sim_step(gmes_sim* s, int64_t nsteps) { return do_step(s, nsteps); }
+int gmes_sim_step(gmes_sim* s, int64_t nsteps) {
+  ApiGuard api;
+  return do_step(s, nsteps);
+}
 
 int gmes_sim_sync(gmes_sim* s) {
+  ApiGuard api;
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(sync_exec(s));
   Ctl c;
@@ -1357,17 +1390,20 @@ int gmes_sim_sync(gmes_sim* s) {
 }
 
 int gmes_sim_get_step(const gmes_sim* s, int64_t* step) {
+  ApiGuard api;
   *step = s->step;
   return GMES_OK;
 }
 
 int gmes_sim_set_graph(gmes_sim* s, int32_t enable) {
+  ApiGuard api;
   s->use_graph = enable != 0;
   destroy_graph(s);
   return GMES_OK;
 }
 
 int gmes_sim_set_ghost_fusion(gmes_sim* s, int32_t enable, int32_t* fused) {
+  ApiGuard api;
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(sync_exec(s));
   s->allow_fuse = enable != 0;
@@ -1377,6 +1413,7 @@ int gmes_sim_set_ghost_fusion(gmes_sim* s, int32_t enable, int32_t* fused) {
 }
 
 int gmes_sim_set_tile(gmes_sim* s, int32_t bx, int32_t by) {
+  ApiGuard api;
   if (bx < 0 || by < 0 || bx * by > 256 || (bx && (bx & (bx - 1))))
     return set_error(GMES_EINVAL, "bad tile: bx must be a power of two and bx*by <= 256");
   s->tile_bx = bx;
@@ -1388,6 +1425,7 @@ int gmes_sim_set_tile(gmes_sim* s, int32_t bx, int32_t by) {
 }
 
 int gmes_sim_autotune(gmes_sim* s, int32_t* bx, int32_t* by) {
+  ApiGuard api;
   if (!s->committed) return set_error(GMES_EINVAL, "sim is not committed");
   HIP_TRY(hipSetDevice(s->device));
   int tbx = 0, tby = 0, rc;
@@ -1409,6 +1447,7 @@ int gmes_sim_autotune(gmes_sim* s, int32_t* bx, int32_t* by) {
 }
 
 int gmes_sim_profile(gmes_sim* s, int32_t enable) {
+  ApiGuard api;
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(sync_exec(s));
   collect_timing(s);
@@ -1418,6 +1457,7 @@ int gmes_sim_profile(gmes_sim* s, int32_t enable) {
 }
 
 int gmes_sim_read_profile(gmes_sim* s, double ms[6], int64_t launches[6]) {
+  ApiGuard api;
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(sync_exec(s));
   int rc = collect_timing(s);
@@ -1590,6 +1630,7 @@ void destroy_batch_graph(gmes_batch* b) {
 extern "C" {
 
 int gmes_batch_create(gmes_sim* const* sims, int32_t n, gmes_batch** out) {
+  ApiGuard api;
   if (n < 1) return set_error(GMES_EINVAL, "empty batch");
   for (int i = 0; i < n; ++i) {
     gmes_sim* s = sims[i];
@@ -1627,6 +1668,7 @@ int gmes_batch_create(gmes_sim* const* sims, int32_t n, gmes_batch** out) {
 }
 
 int gmes_batch_destroy(gmes_batch* b) {
+  ApiGuard api;
   if (!b) return GMES_OK;
   (void)hipSetDevice(b->device);
   (void)hipStreamSynchronize(b->stream);
@@ -1639,6 +1681,7 @@ int gmes_batch_destroy(gmes_batch* b) {
 }
 
 int gmes_batch_set_graph(gmes_batch* b, int32_t enable) {
+  ApiGuard api;
   b->use_graph = enable != 0;
   destroy_batch_graph(b);
   return GMES_OK;
@@ -1647,6 +1690,7 @@ int gmes_batch_set_graph(gmes_batch* b, int32_t enable) {
 // Advance every member by nsteps: one batched launch per kernel kind and
 // half step, replayed from one captured graph per step.
 int gmes_batch_step(gmes_batch* b, int64_t nsteps) {
+  ApiGuard api;
   if (nsteps < 0) return set_error(GMES_EINVAL, "negative step count");
   for (gmes_sim* s : b->sims) {
     if (s->timing.on) return set_error(GMES_EINVAL, "kernel timing is per sim; switch it off for batch members");
@@ -1701,7 +1745,7 @@ int gmes_batch_step(gmes_batch* b, int64_t nsteps) {
     if (!b->graph_exec) {
       hipGraph_t graph;
       {
-        std::unique_lock<std::shared_mutex> l(g_capture_mu);
+        CaptureGuard capture;
         HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         enqueue_batch_step(b, b->stream);
         HIP_TRY(hipStreamEndCapture(b->stream, &graph));
@@ -1720,6 +1764,7 @@ int gmes_batch_step(gmes_batch* b, int64_t nsteps) {
 }
 
 int gmes_batch_sync(gmes_batch* b) {
+  ApiGuard api;
   HIP_TRY(hipSetDevice(b->device));
   HIP_TRY(hipStreamSynchronize(b->stream));
   for (gmes_sim* s : b->sims) {

@@ -37,6 +37,45 @@ template <class R> __device__ __forceinline__ cplx<R> cmul(cplx<R> a, cplx<R> b)
 }
 template <class T> __device__ __forceinline__ T vzero() { return T{}; }
 
+// Global-memory accessors.  A device allocation reached through a pointer
+// read from memory (a batch job table) is not known to be global: plain
+// accesses through it become generic (flat) instructions, which also wait
+// on LDS traffic.  These access it through an address-space-1 pointer.
+#define GMES_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ T gld(const T* p) {
+  return *(const GMES_GLOBAL T*)p;
+}
+template <class T>
+__device__ __forceinline__ void gst(T* p, T v) {
+  *(GMES_GLOBAL T*)p = v;
+}
+// complex words and records move as native vectors (a class-typed copy
+// would take a generic pointer again)
+template <class R>
+using vec2_t = R __attribute__((ext_vector_type(2)));
+typedef int i4_t __attribute__((ext_vector_type(4)));
+template <class R>
+__device__ __forceinline__ cplx<R> gld(const cplx<R>* p) {
+  const vec2_t<R> v = *(const GMES_GLOBAL vec2_t<R>*)p;
+  return {v.x, v.y};
+}
+template <class R>
+__device__ __forceinline__ void gst(cplx<R>* p, cplx<R> v) {
+  *(GMES_GLOBAL vec2_t<R>*)p = vec2_t<R>{v.re, v.im};
+}
+template <class R>
+__device__ __forceinline__ DevRec<R> load_rec(const DevRec<R>* p) {
+  constexpr int n = sizeof(DevRec<R>) / 16;
+  union {
+    DevRec<R> r;
+    i4_t v[n];
+  } u;
+#pragma unroll
+  for (int c = 0; c < n; ++c) u.v[c] = ((const GMES_GLOBAL i4_t*)p)[c];
+  return u.r;
+}
+
 // Streaming accesses: field words and material ids are touched once per
 // launch, so they are loaded and stored non-temporally (no reuse to keep
 // in L2).  GMES_NT=0 builds the plain form.
@@ -45,9 +84,9 @@ template <class T> __device__ __forceinline__ T vzero() { return T{}; }
 #endif
 template <class R> __device__ __forceinline__ R ld_nt(const R* p) {
 #if GMES_NT
-  return __builtin_nontemporal_load(p);
+  return __builtin_nontemporal_load((const __attribute__((address_space(1))) R*)p);
 #else
-  return *p;
+  return *(const __attribute__((address_space(1))) R*)p;
 #endif
 }
 template <class R> __device__ __forceinline__ cplx<R> ld_nt(const cplx<R>* p) {
@@ -55,9 +94,9 @@ template <class R> __device__ __forceinline__ cplx<R> ld_nt(const cplx<R>* p) {
 }
 template <class R> __device__ __forceinline__ void st_nt(R* p, R v) {
 #if GMES_NT
-  __builtin_nontemporal_store(v, p);
+  __builtin_nontemporal_store(v, (__attribute__((address_space(1))) R*)p);
 #else
-  *p = v;
+  *(__attribute__((address_space(1))) R*)p = v;
 #endif
 }
 template <class R> __device__ __forceinline__ void st_nt(cplx<R>* p, cplx<R> v) {
@@ -79,7 +118,7 @@ template <class T>
 __device__ __forceinline__ T ld(const FieldArgs<T>& a, int c, int i, int j, int k) {
   // an inactive component is (1,1,1) and reads as zero (src/cpp23_support.hh:52-74)
   if (!(a.active & (1u << c))) return vzero<T>();
-  return a.f[c][didx(a, c, i, j, k)];
+  return gld(a.f[c] + didx(a, c, i, j, k));
 }
 
 // Curl differences D1, D2 of component C at (i,j,k): src/pw_dielectric.hh
@@ -140,25 +179,25 @@ __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typ
     const T a = vscl(id1, D1), b = vscl(id2, D2);  // D1/d1, D2/d2
     if (cls == GMES_CLS_DIELECTRIC) return E ? fma_(rec.c0, vsub(a, b), f) : fma_(rec.c0, vsub(b, a), f);
     const int ix[3] = {i, j, k};
-    const int64_t* po = m.prof_off + (rec.prof * 6 + C) * 3;
+    const GMES_GLOBAL int64_t* po = (const GMES_GLOBAL int64_t*)m.prof_off + (rec.prof * 6 + C) * 3;
     if (cls == GMES_CLS_CPML) {
-      const float* p1 = m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 3;
-      const float* p2 = m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 3;
-      const T psi1 = fma_(p1[1], a, vscl(p1[0], st[flat]));
-      const T psi2 = fma_(p2[1], b, vscl(p2[0], st[m.dsize[C] + flat]));
-      st[flat] = psi1;
-      st[m.dsize[C] + flat] = psi2;
+      const GMES_GLOBAL float* p1 = (const GMES_GLOBAL float*)m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 3;
+      const GMES_GLOBAL float* p2 = (const GMES_GLOBAL float*)m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 3;
+      const T psi1 = fma_(p1[1], a, vscl(p1[0], gld(st + flat)));
+      const T psi2 = fma_(p2[1], b, vscl(p2[0], gld(st + m.dsize[C] + flat)));
+      gst(st + flat, psi1);
+      gst(st + m.dsize[C] + flat, psi2);
       const T t = vsub(vadd(fma_(-p2[2], b, vscl(p1[2], a)), psi1), psi2);
       return E ? fma_(rec.c0, t, f) : fma_(-rec.c0, t, f);
     }
     // UPML
-    const float* p1 = m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 6;
-    const float* p2 = m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 6;
-    const float* p0 = m.prof + po[Ax<C>::a0] + ix[Ax<C>::a0] * 6;
-    const T dstore = st[flat];
+    const GMES_GLOBAL float* p1 = (const GMES_GLOBAL float*)m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 6;
+    const GMES_GLOBAL float* p2 = (const GMES_GLOBAL float*)m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 6;
+    const GMES_GLOBAL float* p0 = (const GMES_GLOBAL float*)m.prof + po[Ax<C>::a0] + ix[Ax<C>::a0] * 6;
+    const T dstore = gld(st + flat);
     const T cu = vsub(a, b);
     const T dn = E ? fma_(p1[1], cu, vscl(p1[0], dstore)) : fma_(-p1[1], cu, vscl(p1[0], dstore));
-    st[flat] = dn;
+    gst(st + flat, dn);
     return fma_(p2[3] * rec.inv_eps, fma_(-p0[5], dstore, vscl(p0[4], dn)), vscl(p2[2], f));
   } else {
     const R d1 = m.d[Ax<C>::a1], d2 = m.d[Ax<C>::a2];
@@ -167,29 +206,29 @@ __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typ
       else return vadd(f, vscl(rec.c0, vsub(vdiv(D2, d2), vdiv(D1, d1))));
     }
     const int ix[3] = {i, j, k};
-    const int64_t* po = m.prof_off + (rec.prof * 6 + C) * 3;
+    const GMES_GLOBAL int64_t* po = (const GMES_GLOBAL int64_t*)m.prof_off + (rec.prof * 6 + C) * 3;
     if (cls == GMES_CLS_CPML) {
-      const R* p1 = m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 3;
-      const R* p2 = m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 3;
-      T psi1 = st[flat], psi2 = st[m.dsize[C] + flat];
+      const GMES_GLOBAL R* p1 = (const GMES_GLOBAL R*)m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 3;
+      const GMES_GLOBAL R* p2 = (const GMES_GLOBAL R*)m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 3;
+      T psi1 = gld(st + flat), psi2 = gld(st + m.dsize[C] + flat);
       psi1 = vadd(vscl(p1[0], psi1), vdiv(vscl(p1[1], D1), d1));
       psi2 = vadd(vscl(p2[0], psi2), vdiv(vscl(p2[1], D2), d2));
-      st[flat] = psi1;
-      st[m.dsize[C] + flat] = psi2;
+      gst(st + flat, psi1);
+      gst(st + m.dsize[C] + flat, psi2);
       const T t = vsub(vadd(vsub(vdiv(vdiv(D1, d1), p1[2]), vdiv(vdiv(D2, d2), p2[2])), psi1), psi2);
       if constexpr (E) return vadd(f, vscl(rec.c0, t));
       else return vsub(f, vscl(rec.c0, t));
     }
     // UPML
-    const R* p1 = m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 6;
-    const R* p2 = m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 6;
-    const R* p0 = m.prof + po[Ax<C>::a0] + ix[Ax<C>::a0] * 6;
-    const T dstore = st[flat];
+    const GMES_GLOBAL R* p1 = (const GMES_GLOBAL R*)m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 6;
+    const GMES_GLOBAL R* p2 = (const GMES_GLOBAL R*)m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 6;
+    const GMES_GLOBAL R* p0 = (const GMES_GLOBAL R*)m.prof + po[Ax<C>::a0] + ix[Ax<C>::a0] * 6;
+    const T dstore = gld(st + flat);
     const T cu = vsub(vdiv(D1, d1), vdiv(D2, d2));
     T dn;
     if constexpr (E) dn = vadd(vscl(p1[0], dstore), vscl(p1[1], cu));
     else dn = vsub(vscl(p1[0], dstore), vscl(p1[1], cu));
-    st[flat] = dn;
+    gst(st + flat, dn);
     return vadd(vscl(p2[2], f), vdiv(vscl(p2[3], vsub(vscl(p0[4], dn), vscl(p0[5], dstore))), rec.eps_mu));
   }
 }
@@ -237,15 +276,15 @@ __device__ __forceinline__ void fused_ghost(const MainArgs<T>& m, int C, T* __re
   if (on1) {
     x1 = ghost_phase<T>(m.gph[C][0], x);
     g[a1] = dst1;
-    F[didx(m.fa, C, g[0], g[1], g[2])] = x1;
+    gst(F + didx(m.fa, C, g[0], g[1], g[2]), x1);
     g[a1] = 0;
   }
   if (on2) {
     g[a2] = dst2;
-    F[didx(m.fa, C, g[0], g[1], g[2])] = ghost_phase<T>(m.gph[C][1], x);
+    gst(F + didx(m.fa, C, g[0], g[1], g[2]), ghost_phase<T>(m.gph[C][1], x));
     if (on1) {
       g[a1] = dst1;
-      F[didx(m.fa, C, g[0], g[1], g[2])] = ghost_phase<T>(m.gph[C][1], x1);
+      gst(F + didx(m.fa, C, g[0], g[1], g[2]), ghost_phase<T>(m.gph[C][1], x1));
     }
   }
 }
@@ -257,14 +296,14 @@ __device__ __forceinline__ void update_cell(const MainArgs<T>& m, const DevRec<t
   if (cls == GMES_CLS_DUMMY || cls >= GMES_CLS_DRUDE) return;
   T* F = m.fa.f[C];
   (void)store_cell<C>(m, rec, F, static_cast<T*>(m.pml[C]), flat, i, j, k,
-                      cls == GMES_CLS_CONST ? vzero<T>() : F[flat], D1, D2);
+                      cls == GMES_CLS_CONST ? vzero<T>() : gld(F + flat), D1, D2);
 }
 
 template <int C, class T>
 __device__ __forceinline__ void main_cell(const MainArgs<T>& m, int i, int j, int k) {
   const FieldArgs<T>& a = m.fa;
   const int flat = didx(a, C, i, j, k);
-  const auto& rec = m.recs[m.ids[C][flat]];
+  const DevRec<typename real_of<T>::type> rec = load_rec(m.recs + gld(m.ids[C] + flat));
   if (rec.cls == GMES_CLS_DUMMY || rec.cls >= GMES_CLS_DRUDE) return;
   T D1 = vzero<T>(), D2 = vzero<T>();
   if (rec.cls != GMES_CLS_CONST) curl_terms<C>(a, i, j, k, D1, D2);
@@ -317,13 +356,13 @@ __device__ __forceinline__ void load_e(Planes<T, N>& q, int i0, int j, int k, co
 #pragma unroll
   for (int n = 0; n < N; ++n) {
     const int i = i0 + n;
-    q.c[0][n] = hx[B[3] + i * P[3] + (j + 1) * W[3] + k + 1];  // hx(i, j+1, k+1)
-    q.c[1][n] = hx[B[3] + i * P[3] + j * W[3] + k + 1];        // hx(i, j,   k+1)
-    q.c[2][n] = hx[B[3] + i * P[3] + (j + 1) * W[3] + k];      // hx(i, j+1, k)
-    q.c[3][n] = hy[B[4] + (i + 1) * P[4] + j * W[4] + k + 1];  // hy(i+1, j, k+1)
-    q.c[4][n] = hy[B[4] + (i + 1) * P[4] + j * W[4] + k];      // hy(i+1, j, k)
-    q.c[5][n] = hz[B[5] + (i + 1) * P[5] + (j + 1) * W[5] + k];  // hz(i+1, j+1, k)
-    q.c[6][n] = hz[B[5] + (i + 1) * P[5] + j * W[5] + k];        // hz(i+1, j,   k)
+    q.c[0][n] = gld(hx + B[3] + i * P[3] + (j + 1) * W[3] + k + 1);  // hx(i, j+1, k+1)
+    q.c[1][n] = gld(hx + B[3] + i * P[3] + j * W[3] + k + 1);        // hx(i, j,   k+1)
+    q.c[2][n] = gld(hx + B[3] + i * P[3] + (j + 1) * W[3] + k);      // hx(i, j+1, k)
+    q.c[3][n] = gld(hy + B[4] + (i + 1) * P[4] + j * W[4] + k + 1);  // hy(i+1, j, k+1)
+    q.c[4][n] = gld(hy + B[4] + (i + 1) * P[4] + j * W[4] + k);      // hy(i+1, j, k)
+    q.c[5][n] = gld(hz + B[5] + (i + 1) * P[5] + (j + 1) * W[5] + k);  // hz(i+1, j+1, k)
+    q.c[6][n] = gld(hz + B[5] + (i + 1) * P[5] + j * W[5] + k);        // hz(i+1, j,   k)
     const int fx = B[0] + i * P[0] + j * W[0] + k, fy = B[1] + i * P[1] + j * W[1] + k;
     const int fz = B[2] + i * P[2] + j * W[2] + k;
     q.o[0][n] = ld_nt(ex + fx);
@@ -374,13 +413,13 @@ __device__ __forceinline__ void load_h(Planes<T, N>& q, int i0, int j, int k, co
 #pragma unroll
   for (int n = 0; n < N; ++n) {
     const int i = i0 + n;
-    q.c[0][n] = ex[B[0] + i * P[0] + j * W[0] + k];            // ex(i, j, k)
-    q.c[1][n] = ex[B[0] + i * P[0] + j * W[0] + k + 1];        // ex(i, j, k+1)
-    q.c[2][n] = ex[B[0] + i * P[0] + (j + 1) * W[0] + k];      // ex(i, j+1, k)
-    q.c[3][n] = ey[B[1] + i * P[1] + j * W[1] + k + 1];        // ey(i, j, k+1)
-    q.c[4][n] = ey[B[1] + (i + 1) * P[1] + j * W[1] + k];      // ey(i+1, j, k)
-    q.c[5][n] = ez[B[2] + i * P[2] + (j + 1) * W[2] + k];      // ez(i, j+1, k)
-    q.c[6][n] = ez[B[2] + (i + 1) * P[2] + j * W[2] + k];      // ez(i+1, j, k)
+    q.c[0][n] = gld(ex + B[0] + i * P[0] + j * W[0] + k);            // ex(i, j, k)
+    q.c[1][n] = gld(ex + B[0] + i * P[0] + j * W[0] + k + 1);        // ex(i, j, k+1)
+    q.c[2][n] = gld(ex + B[0] + i * P[0] + (j + 1) * W[0] + k);      // ex(i, j+1, k)
+    q.c[3][n] = gld(ey + B[1] + i * P[1] + j * W[1] + k + 1);        // ey(i, j, k+1)
+    q.c[4][n] = gld(ey + B[1] + (i + 1) * P[1] + j * W[1] + k);      // ey(i+1, j, k)
+    q.c[5][n] = gld(ez + B[2] + i * P[2] + (j + 1) * W[2] + k);      // ez(i, j+1, k)
+    q.c[6][n] = gld(ez + B[2] + (i + 1) * P[2] + j * W[2] + k);      // ez(i+1, j, k)
     const int fx = B[3] + i * P[3] + (j + 1) * W[3] + k + 1;    // Hx(i, j+1, k+1)
     const int fy = B[4] + (i + 1) * P[4] + j * W[4] + k + 1;    // Hy(i+1, j, k+1)
     const int fz = B[5] + (i + 1) * P[5] + (j + 1) * W[5] + k;  // Hz(i+1, j+1, k)
@@ -423,26 +462,28 @@ template <class R>
 struct RecStage {
   static constexpr int kChunks = sizeof(DevRec<R>) / 16;
   static_assert(sizeof(DevRec<R>) % 16 == 0, "record size must be a multiple of 16 bytes");
-  int4 v;
+  i4_t v;
   int t;
   bool use_lds;
   __device__ __forceinline__ void load(const DevRec<R>* g, int nrec, int tid, int nthreads) {
     t = tid;
     use_lds = nrec * kChunks <= nthreads && nrec <= kRecLds;
-    if (use_lds && t < nrec * kChunks) v = reinterpret_cast<const int4*>(g)[t];
+    if (use_lds && t < nrec * kChunks) v = gld(reinterpret_cast<const i4_t*>(g) + t);
   }
-  __device__ __forceinline__ const DevRec<R>* commit(DevRec<R>* lds, const DevRec<R>* g, int nrec) {
-    if (!use_lds) return g;
-    if (t < nrec * kChunks) reinterpret_cast<int4*>(lds)[t] = v;
+  __device__ __forceinline__ void commit(DevRec<R>* lds, int nrec) {
+    if (!use_lds) return;
+    if (t < nrec * kChunks) reinterpret_cast<i4_t*>(lds)[t] = v;
     __syncthreads();
-    return lds;
   }
 };
 
 // One tile of the 3-D update: thread (k, j) of planes i0 .. i0+N-1; tid and
 // nthreads are the thread's rank in the block and the block size (the
 // record-table staging and its barrier involve every thread of the block).
-template <bool E, class T, int N>
+// SPLIT: separate update code for records in LDS and in global memory
+// (faster for the single-sim kernel); otherwise one copy reads them
+// through a generic pointer (faster, as measured, for the batched kernel).
+template <bool E, class T, int N, bool SPLIT>
 __device__ __forceinline__ void main3d_tile(const MainArgs<T>& m, DevRec<typename real_of<T>::type>* srec, int k,
                                             int j, int i0, int tid, int nthreads) {
   using R = typename real_of<T>::type;
@@ -459,36 +500,47 @@ __device__ __forceinline__ void main3d_tile(const MainArgs<T>& m, DevRec<typenam
   const uint16_t* ida = m.ids[E ? 0 : 3];
   const uint16_t* idb = m.ids[E ? 1 : 4];
   const uint16_t* idc = m.ids[E ? 2 : 5];
+  const DevRec<R>* grec = m.recs;
   RecStage<R> stage;
-  stage.load(m.recs, m.nrec, tid, nthreads);
+  stage.load(grec, m.nrec, tid, nthreads);
   if (i0 + N <= nx) {
     Planes<T, N> q;
     T u0 = T{}, u1 = T{};
     if (act) {
       if constexpr (E) {
         load_e<T, N>(q, i0, j, k, B, P, W, f[3], f[4], f[5], f[0], f[1], f[2], ida, idb, idc);
-        u0 = f[5][B[5] + i0 * P[5] + (j + 1) * W[5] + k];
-        u1 = f[4][B[4] + i0 * P[4] + j * W[4] + k + 1];
+        u0 = gld(f[5] + B[5] + i0 * P[5] + (j + 1) * W[5] + k);
+        u1 = gld(f[4] + B[4] + i0 * P[4] + j * W[4] + k + 1);
       } else {
         load_h<T, N>(q, i0, j, k, B, P, W, f[0], f[1], f[2], f[3], f[4], f[5], ida, idb, idc);
-        u0 = f[1][B[1] + i0 * P[1] + j * W[1] + k];
-        u1 = f[2][B[2] + i0 * P[2] + j * W[2] + k];
+        u0 = gld(f[1] + B[1] + i0 * P[1] + j * W[1] + k);
+        u1 = gld(f[2] + B[2] + i0 * P[2] + j * W[2] + k);
       }
     }
-    const DevRec<R>* recs = stage.commit(srec, m.recs, m.nrec);
+    stage.commit(srec, m.nrec);
     if (!act) return;
-    if constexpr (E) update_e<T, N>(m, recs, q, i0, j, k, B, P, W, f[0], f[1], f[2], sx, sy, sz, u0, u1);
-    else update_h<T, N>(m, recs, q, i0, j, k, B, P, W, f[3], f[4], f[5], sx, sy, sz, u0, u1);
+    if constexpr (!SPLIT) {
+      const DevRec<R>* recs = stage.use_lds ? srec : grec;
+      if constexpr (E) update_e<T, N>(m, recs, q, i0, j, k, B, P, W, f[0], f[1], f[2], sx, sy, sz, u0, u1);
+      else update_h<T, N>(m, recs, q, i0, j, k, B, P, W, f[3], f[4], f[5], sx, sy, sz, u0, u1);
+    } else if (stage.use_lds) {
+      if constexpr (E) update_e<T, N>(m, srec, q, i0, j, k, B, P, W, f[0], f[1], f[2], sx, sy, sz, u0, u1);
+      else update_h<T, N>(m, srec, q, i0, j, k, B, P, W, f[3], f[4], f[5], sx, sy, sz, u0, u1);
+    } else {
+      if constexpr (E) update_e<T, N>(m, grec, q, i0, j, k, B, P, W, f[0], f[1], f[2], sx, sy, sz, u0, u1);
+      else update_h<T, N>(m, grec, q, i0, j, k, B, P, W, f[3], f[4], f[5], sx, sy, sz, u0, u1);
+    }
   } else {
-    const DevRec<R>* recs = stage.commit(srec, m.recs, m.nrec);
+    stage.commit(srec, m.nrec);
     if (!act) return;
+    const DevRec<R>* recs = stage.use_lds ? srec : grec;  // remainder planes only
     T u0, u1;
     if constexpr (E) {
-      u0 = f[5][B[5] + i0 * P[5] + (j + 1) * W[5] + k];
-      u1 = f[4][B[4] + i0 * P[4] + j * W[4] + k + 1];
+      u0 = gld(f[5] + B[5] + i0 * P[5] + (j + 1) * W[5] + k);
+      u1 = gld(f[4] + B[4] + i0 * P[4] + j * W[4] + k + 1);
     } else {
-      u0 = f[1][B[1] + i0 * P[1] + j * W[1] + k];
-      u1 = f[2][B[2] + i0 * P[2] + j * W[2] + k];
+      u0 = gld(f[1] + B[1] + i0 * P[1] + j * W[1] + k);
+      u1 = gld(f[2] + B[2] + i0 * P[2] + j * W[2] + k);
     }
     for (int i = i0; i < nx; ++i) {
       Planes<T, 1> q;
@@ -506,7 +558,7 @@ __device__ __forceinline__ void main3d_tile(const MainArgs<T>& m, DevRec<typenam
 template <bool E, class T, int N>
 __global__ void __launch_bounds__(256) k_main3d(MainArgs<T> m) {
   __shared__ DevRec<typename real_of<T>::type> srec[kRecLds];
-  main3d_tile<E, T, N>(m, srec, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
+  main3d_tile<E, T, N, true>(m, srec, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
                        blockIdx.z * N, threadIdx.y * blockDim.x + threadIdx.x, blockDim.x * blockDim.y);
 }
 
@@ -525,7 +577,7 @@ __global__ void __launch_bounds__(256) k_main3d_batch(const MainJob<T>* jobs, in
   const int bxi = local % J.gx, byi = (local / J.gx) % J.gy, bzi = local / (J.gx * J.gy);
   const int tid = threadIdx.x;
   const int tx = tid & (J.bx - 1), ty = tid / J.bx;
-  main3d_tile<E, T, 1>(J.m, srec, bxi * J.bx + tx, byi * (256 / J.bx) + ty, bzi, tid, 256);
+  main3d_tile<E, T, 1, false>(J.m, srec, bxi * J.bx + tx, byi * (256 / J.bx) + ty, bzi, tid, 256);
 }
 
 template <bool E, class T>
@@ -933,7 +985,7 @@ __device__ __forceinline__ void ghost_point(const GhostArgs<T>& m, int64_t e) {
     int sx[3] = {ix[0], ix[1], ix[2]};
     sx[a1] = src1;
     if (ix[a2] == dst2) sx[a2] = src2;
-    v = f[didx(m.fa, c, sx[0], sx[1], sx[2])];
+    v = gld(f + didx(m.fa, c, sx[0], sx[1], sx[2]));
     if constexpr (is_cplx<T>::value) {
       v = cmul(m.ph[t][0], v);
       if (ix[a2] == dst2) v = cmul(m.ph[t][1], v);
@@ -953,10 +1005,10 @@ __device__ __forceinline__ void ghost_point(const GhostArgs<T>& m, int64_t e) {
     ix[a2] = dst2;
     int sx[3] = {ix[0], ix[1], ix[2]};
     sx[a2] = src2;
-    v = f[didx(m.fa, c, sx[0], sx[1], sx[2])];
+    v = gld(f + didx(m.fa, c, sx[0], sx[1], sx[2]));
     if constexpr (is_cplx<T>::value) v = cmul(m.ph[t][1], v);
   }
-  f[didx(m.fa, c, ix[0], ix[1], ix[2])] = v;
+  gst(f + didx(m.fa, c, ix[0], ix[1], ix[2]), v);
 }
 
 template <class T>
