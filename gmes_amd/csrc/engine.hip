@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -139,6 +140,32 @@ void free_ade(gmes_sim* s) {
     }
 }
 
+// Coordinates along `axis` where the CPML c coefficient of profile set p is
+// exactly zero for every active component: the longest such run [lo, hi),
+// packed as lo | hi << 16 (0 = empty: every coordinate keeps its psi).
+// There psi stays +0, and the kernels neither read nor write it.
+int32_t zero_c_run(const gmes_sim* s, int p, int axis) {
+  int lo = 0, hi = INT32_MAX;
+  for (int c = 0; c < 6; ++c) {
+    if (!active(s, c)) continue;
+    const std::vector<double>& v = s->prof_host[p][c][axis];
+    if (s->prof_w[p][c][axis] != 3 || v.empty()) return 0;
+    const int n = (int)(v.size() / 3);
+    int best_a = 0, best_b = 0;
+    for (int q = 0; q < n;) {
+      if (v[3 * (size_t)q + 1] != 0.0) { ++q; continue; }
+      int e = q;
+      while (e < n && v[3 * (size_t)e + 1] == 0.0) ++e;
+      if (e - q > best_b - best_a) { best_a = q; best_b = e; }
+      q = e;
+    }
+    lo = std::max(lo, best_a);
+    hi = std::min(hi, best_b);
+  }
+  if (lo >= hi || hi > 0x7FFF) return 0;
+  return lo | (hi << 16);
+}
+
 template <class R>
 int upload_tables(gmes_sim* s) {
   std::vector<DevRec<R>> dr(s->recs.size());
@@ -149,6 +176,11 @@ int upload_tables(gmes_sim* s) {
     dr[i].n2 = r.n2;
     dr[i].prof = r.prof;
     dr[i].coef_off = r.coef_off;
+    if (r.cls == GMES_CLS_CPML && r.prof >= 0 && r.prof < s->nprof) {
+      dr[i].n1 = zero_c_run(s, r.prof, 0);
+      dr[i].n2 = zero_c_run(s, r.prof, 1);
+      dr[i].coef_off = zero_c_run(s, r.prof, 2);
+    }
     dr[i].c0 = (R)(s->g.dt / r.eps_mu);  // dt / eps_inf in FP64, rounded once
     dr[i].eps_mu = (R)r.eps_mu;
     dr[i].inv_eps = (R)(1.0 / r.eps_mu);

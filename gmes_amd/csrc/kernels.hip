@@ -160,6 +160,19 @@ __device__ __forceinline__ cplx<float> fma_(float a, cplx<float> b, cplx<float> 
   return {__builtin_fmaf(a, b.re, c.re), __builtin_fmaf(a, b.im, c.im)};
 }
 
+// CPML psi words that can be nonzero.  Along an axis where the profile's
+// c coefficient is exactly zero (outside that axis's layers: sigma = 0),
+// psi starts at +0 and stays +0, since b*(+0) + (+-0)*x = +0; computing it
+// from a literal +0 gives the same value, and the word is neither read nor
+// written.  For a CPML record the engine packs, per axis, the coordinate
+// run [lo, hi) where c is zero for every component as lo | hi << 16, into
+// n1 (x), n2 (y) and coef_off (z), fields CPML records do not otherwise use.
+template <int A, class R>
+__device__ __forceinline__ bool psi_live(const DevRec<R>& rec, const int ix[3]) {
+  const int run = A == 0 ? rec.n1 : A == 1 ? rec.n2 : (int)rec.coef_off;
+  return ix[A] < (run & 0xFFFF) || ix[A] >= (run >> 16);
+}
+
 // New value of a Dielectric, CPML or UPML cell of component C from its old
 // value f and curl differences D1, D2.  st is the component's CPML/UPML
 // state array (same layout as the field), flat the cell's element.
@@ -183,10 +196,17 @@ __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typ
     if (cls == GMES_CLS_CPML) {
       const GMES_GLOBAL float* p1 = (const GMES_GLOBAL float*)m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 3;
       const GMES_GLOBAL float* p2 = (const GMES_GLOBAL float*)m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 3;
-      const T psi1 = fma_(p1[1], a, vscl(p1[0], gld(st + flat)));
-      const T psi2 = fma_(p2[1], b, vscl(p2[0], gld(st + m.dsize[C] + flat)));
-      gst(st + flat, psi1);
-      gst(st + m.dsize[C] + flat, psi2);
+      // real FP32: the branches around the psi words cost more than their
+      // traffic (measured), so every CPML cell moves both
+      constexpr bool kElide = sizeof(T) >= 8;
+      const bool u1 = !kElide || psi_live<Ax<C>::a1>(rec, ix);
+      const bool u2 = !kElide || psi_live<Ax<C>::a2>(rec, ix);
+      const T o1 = u1 ? gld(st + flat) : vzero<T>();
+      const T o2 = u2 ? gld(st + m.dsize[C] + flat) : vzero<T>();
+      const T psi1 = fma_(p1[1], a, vscl(p1[0], o1));
+      const T psi2 = fma_(p2[1], b, vscl(p2[0], o2));
+      if (u1) gst(st + flat, psi1);
+      if (u2) gst(st + m.dsize[C] + flat, psi2);
       const T t = vsub(vadd(fma_(-p2[2], b, vscl(p1[2], a)), psi1), psi2);
       return E ? fma_(rec.c0, t, f) : fma_(-rec.c0, t, f);
     }
@@ -210,11 +230,13 @@ __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typ
     if (cls == GMES_CLS_CPML) {
       const GMES_GLOBAL R* p1 = (const GMES_GLOBAL R*)m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 3;
       const GMES_GLOBAL R* p2 = (const GMES_GLOBAL R*)m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 3;
-      T psi1 = gld(st + flat), psi2 = gld(st + m.dsize[C] + flat);
-      psi1 = vadd(vscl(p1[0], psi1), vdiv(vscl(p1[1], D1), d1));
-      psi2 = vadd(vscl(p2[0], psi2), vdiv(vscl(p2[1], D2), d2));
-      gst(st + flat, psi1);
-      gst(st + m.dsize[C] + flat, psi2);
+      const bool u1 = psi_live<Ax<C>::a1>(rec, ix), u2 = psi_live<Ax<C>::a2>(rec, ix);
+      const T o1 = u1 ? gld(st + flat) : vzero<T>();
+      const T o2 = u2 ? gld(st + m.dsize[C] + flat) : vzero<T>();
+      const T psi1 = vadd(vscl(p1[0], o1), vdiv(vscl(p1[1], D1), d1));
+      const T psi2 = vadd(vscl(p2[0], o2), vdiv(vscl(p2[1], D2), d2));
+      if (u1) gst(st + flat, psi1);
+      if (u2) gst(st + m.dsize[C] + flat, psi2);
       const T t = vsub(vadd(vsub(vdiv(vdiv(D1, d1), p1[2]), vdiv(vdiv(D2, d2), p2[2])), psi1), psi2);
       if constexpr (E) return vadd(f, vscl(rec.c0, t));
       else return vsub(f, vscl(rec.c0, t));

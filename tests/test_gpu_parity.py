@@ -260,3 +260,18 @@ def test_ghost_fusion_off_for_dispersive_source_planes(engine_lib):
     """The dispersive Bloch cell's DCP slab covers the x and y source planes:
     its E ghosts keep the separate copy."""
     assert not engine_lib.set_ghost_fusion(_c3(engine_lib)._sim, 1)
+
+
+@pytest.mark.parametrize("builder,kw", [(spheres3d, dict(n=16, nsph=3, pml=4)), (bloch3d, {})],
+                         ids=["spheres3d_cpml_all_faces", "bloch3d_cpml_z"])
+def test_cpml_psi_elision_is_bitwise(engine_lib, oracle_lib, builder, kw):
+    """FP64 / complex128 CPML runs, where the kernels skip the psi words
+    that stay zero (outside each axis's layers), equal the oracle bit for
+    bit, corners and edges of the layers included."""
+    a, b = builder(engine_lib, **kw), builder(oracle_lib, **kw)
+    a._advance(40)
+    b._advance(40)
+    fa, fb = fields(a), fields(b)
+    assert max(np.abs(v).max() for v in fb.values()) > 0
+    for c in fa:
+        np.testing.assert_array_equal(fa[c], fb[c])
