@@ -24,11 +24,15 @@ Also measured, as extra fields of the same line:
   the continuous-batching request loop; p50/p99 latency (completion minus
   arrival) beside aggregate Mcell-updates/s.  A bounded sample of 48
   requests keeps the run to about a minute.
+* ``air2d``: the plain FP64 method, 2-D TMz 200 x 200 in air with a CPML
+  shell run to t = 10 (286 steps), in time per step on the GPU and with
+  the reference's own updaters on the host.
 
 Printed as one JSON line on stdout; diagnostics go to stderr.
 """
 
 import argparse
+import contextlib
 import json
 import os
 import sys
@@ -170,6 +174,42 @@ def dispersive_leg(steps, warm=10):
     return dict(cells=cells, steps=steps, value=round(value, 1), unit="Mcell-updates/s",
                 ms_per_step=round(secs / steps * 1e3, 4), dtype="c64", setup_s=round(setup, 1),
                 ms_per_launch=split)
+
+
+def air2d_leg():
+    """The plain FP64 method: 2-D TMz in air, 200 x 200 cells, run to t = 10
+    (286 steps, examples/air2d.py of the reference), on the GPU and with the
+    reference's own updaters on the host.  The grid is cache resident, so
+    this reports time per step rather than a bandwidth."""
+    from gmes_amd import workloads
+    from gmes_amd._abi import Library
+    out = {}
+    for name, lib in (("gpu", None), ("cpu", "reference")):
+        engine = None
+        if lib:
+            path = os.path.join(ROOT, "oracle", "_ref", "libgmes_ref.so")
+            if not os.path.exists(path):
+                path = os.path.join(ROOT, "oracle", "libgmes_oracle.so")
+                lib = "port"
+            engine = Library(path)
+        sim = workloads.air2d(precision="fp64", _engine=engine)
+        if engine is None:
+            sim._lib.set_graph(sim._sim, 1)
+        cells = int(np.prod(sim.space.whole_field_size))
+        t0 = time.perf_counter()
+        with contextlib.redirect_stdout(sys.stderr):  # its progress lines
+            sim.step_until_t(10)
+        sim._lib.sync(sim._sim)
+        secs = time.perf_counter() - t0
+        steps = int(sim.time_step.n)
+        out[name] = dict(us_per_step=round(secs / steps * 1e6, 2), steps=steps,
+                         value=round(cells * steps / secs / 1e6, 1), unit="Mcell-updates/s")
+        if lib:
+            out[name]["kind"] = lib
+        sim.close()
+    log("air2d: gpu %.1f us/step, cpu %.1f us/step (%d steps)"
+        % (out["gpu"]["us_per_step"], out["cpu"]["us_per_step"], out["gpu"]["steps"]))
+    return out
 
 
 def served_leg(nreq, budget_gb=64.0, max_active=16, chunk=250):
@@ -323,6 +363,7 @@ def main():
         batched = batched_leg(args.batch_steps)
 
     # ---- dispersive Bloch cell --------------------------------------------
+    air2d = None if args.no_cpu else air2d_leg()
     dispersive = None
     if args.dispersive_steps > 0:
         dispersive = dispersive_leg(args.dispersive_steps)
@@ -373,6 +414,7 @@ def main():
         "fp64": fp64,
         "batched": batched,
         "dispersive": dispersive,
+        "air2d": air2d,
         "served": served,
         "roofline": roofline,
         "cpu_baseline": cpu,
