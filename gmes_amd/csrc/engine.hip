@@ -538,6 +538,7 @@ void enqueue_step(gmes_sim* s, hipStream_t st);
 // half steps of s.
 void enqueue_aux_steps(gmes_sim* s, hipStream_t st) {
   for (int i = 0; i < s->naux; ++i) {
+    if (!s->aux[i]) continue;
     time_begin(s, 5, st);
     enqueue_step(s->aux[i], st);
     time_end(s, st);
@@ -571,7 +572,8 @@ void enqueue_step(gmes_sim* s, hipStream_t st) {
 void advance_host_clock(gmes_sim* s, int64_t n) {
   s->step += n;
   s->probe_n = s->step - s->probe_base;
-  for (int i = 0; i < s->naux; ++i) advance_host_clock(s->aux[i], n);
+  for (int i = 0; i < s->naux; ++i)
+    if (s->aux[i]) advance_host_clock(s->aux[i], n);
 }
 
 // Waits for all work that may touch s: its own stream, for an aux sim the
@@ -598,6 +600,7 @@ int check_windows(gmes_sim* s, int64_t n) {
                      (long long)s->dft_step0, (long long)(s->dft_step0 + s->dft_n), (long long)s->step,
                      (long long)(s->step + n));
   for (int i = 0; i < s->naux; ++i) {
+    if (!s->aux[i]) continue;
     int rc = check_windows(s->aux[i], n);
     if (rc) return rc;
   }
@@ -606,6 +609,7 @@ int check_windows(gmes_sim* s, int64_t n) {
 
 int ensure_probe_capacity(gmes_sim* s, int64_t n) {
   for (int i = 0; i < s->naux; ++i) {
+    if (!s->aux[i]) continue;
     int rc = ensure_probe_capacity(s->aux[i], n);
     if (rc) return rc;
   }
@@ -683,6 +687,7 @@ int do_step(gmes_sim* s, int64_t n) {
   if (!s->committed) return set_error(GMES_EINVAL, "sim is not committed");
   if (n < 0) return set_error(GMES_EINVAL, "negative step count");
   if (s->parent) return set_error(GMES_EINVAL, "an auxiliary grid is advanced by its parent");
+  if (s->aux_lost) return set_error(GMES_EINVAL, "an auxiliary grid of this sim was destroyed");
   if (s->batch_stream) HIP_TRY(hipStreamSynchronize(s->batch_stream));
   int rc = check_windows(s, n);
   if (rc) return rc;
@@ -907,10 +912,22 @@ int gmes_sim_create(const gmes_grid_desc* desc, gmes_sim** out) {
 int gmes_sim_destroy(gmes_sim* s) {
   ApiGuard api;
   if (!s) return GMES_OK;
+  // a batch's job tables and graph hold this sim's buffers
+  if (s->batch_stream) return set_error(GMES_EINVAL, "sim is a batch member: destroy the batch first");
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  // an aux sim is advanced on its parent's stream: drain that too
-  if (s->parent && s->parent->stream) (void)hipStreamSynchronize(s->parent->stream);
+  // an aux sim is advanced on its parent's stream: drain that too.  The
+  // parent's source operations read this grid, so the parent refuses to
+  // step from now on.
+  if (gmes_sim* p = s->parent) {
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    if (p->batch_stream) (void)hipStreamSynchronize(p->batch_stream);
+    for (int i = 0; i < p->naux; ++i)
+      if (p->aux[i] == s) p->aux[i] = nullptr;
+    p->aux_lost = true;
+    ++p->gen;
+    destroy_graph(p);
+  }
   for (int i = 0; i < s->naux; ++i)
     if (s->aux[i]) s->aux[i]->parent = nullptr;
   destroy_graph(s);
@@ -935,6 +952,7 @@ int gmes_sim_destroy(gmes_sim* s) {
 
 int gmes_sim_shape(const gmes_sim* s, int32_t c, int64_t shape[3]) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   if (c < 0 || c > 5) return set_error(GMES_EINVAL, "bad component");
   for (int a = 0; a < 3; ++a) shape[a] = s->shape[c][a];
   return GMES_OK;
@@ -942,6 +960,7 @@ int gmes_sim_shape(const gmes_sim* s, int32_t c, int64_t shape[3]) {
 
 int gmes_sim_footprint(const gmes_sim* s, int64_t* bytes) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   *bytes = s->bytes;
   return GMES_OK;
 }
@@ -949,6 +968,7 @@ int gmes_sim_footprint(const gmes_sim* s, int64_t* bytes) {
 int gmes_sim_set_materials(gmes_sim* s, const gmes_mat_rec* recs, int32_t nrec, const double* coef,
                            int64_t ncoef) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   if (nrec < 1 || nrec > 65536) return set_error(GMES_EINVAL, "record count must be in [1, 65536]");
   if (ncoef < 0) return set_error(GMES_EINVAL, "negative coefficient count");
   s->recs.assign(recs, recs + nrec);
@@ -981,6 +1001,7 @@ int gmes_sim_set_materials(gmes_sim* s, const gmes_mat_rec* recs, int32_t nrec, 
 
 int gmes_sim_set_material_ids(gmes_sim* s, int32_t c, const uint16_t* ids, int64_t count) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   if (c < 0 || c > 5) return set_error(GMES_EINVAL, "bad component");
   if (count != s->ncell[c]) return set_error(GMES_EINVAL, "id count %lld != cells %lld", (long long)count,
                                              (long long)s->ncell[c]);
@@ -1004,6 +1025,7 @@ int gmes_sim_set_material_ids(gmes_sim* s, int32_t c, const uint16_t* ids, int64
 int gmes_sim_set_profile(gmes_sim* s, int32_t p, int32_t c, int32_t axis, const double* vals, int32_t n,
                          int32_t width) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   if (p < 0 || p >= 64 || c < 0 || c > 5 || axis < 0 || axis > 2)
     return set_error(GMES_EINVAL, "bad profile address");
   if (n != s->shape[c][axis] || (width != 3 && width != 6)) return set_error(GMES_EINVAL, "bad profile size");
@@ -1016,6 +1038,7 @@ int gmes_sim_set_profile(gmes_sim* s, int32_t p, int32_t c, int32_t axis, const 
 
 int gmes_sim_commit(gmes_sim* s) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   if (s->recs.empty()) return set_error(GMES_EINVAL, "materials not set");
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(sync_exec(s));
@@ -1120,6 +1143,7 @@ int gmes_sim_commit(gmes_sim* s) {
 
 int gmes_sim_set_sources(gmes_sim* s, const gmes_src_op* ops, int64_t nops, int32_t nwaves) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   if (nops < 0 || nwaves < 0) return set_error(GMES_EINVAL, "negative count");
   if (nops > (1 << 30)) return set_error(GMES_EINVAL, "too many source operations");
   for (int64_t i = 0; i < nops; ++i) {
@@ -1152,6 +1176,7 @@ int gmes_sim_set_sources(gmes_sim* s, const gmes_src_op* ops, int64_t nops, int3
 
 int gmes_sim_set_waveforms(gmes_sim* s, int64_t step0, int64_t nsteps, const double* values) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   if (nsteps < 0) return set_error(GMES_EINVAL, "negative step count");
   HIP_TRY(hipSetDevice(s->device));
   const int64_t need = std::max<int64_t>(1, (int64_t)s->nwaves * nsteps * 2);
@@ -1173,6 +1198,7 @@ int gmes_sim_set_waveforms(gmes_sim* s, int64_t step0, int64_t nsteps, const dou
 
 int gmes_sim_attach_aux(gmes_sim* s, gmes_sim* aux, int32_t* slot) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   if (!aux || aux == s) return set_error(GMES_EINVAL, "bad auxiliary sim");
   if (s->naux >= 8) return set_error(GMES_EINVAL, "too many auxiliary grids");
   if (aux->g.dtype != s->g.dtype) return set_error(GMES_ETYPE, "aux dtype mismatch");
@@ -1192,6 +1218,7 @@ int gmes_sim_attach_aux(gmes_sim* s, gmes_sim* aux, int32_t* slot) {
 
 int gmes_sim_add_probe(gmes_sim* s, int32_t c, int64_t cell, int32_t* id) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   if (c < 0 || c > 5 || !active(s, c) || cell < 0 || cell >= s->ncell[c])
     return set_error(GMES_ERANGE, "probe index out of bounds");
   if (s->step != s->probe_base) return set_error(GMES_EINVAL, "read pending probe values first");
@@ -1216,20 +1243,13 @@ int gmes_sim_add_probe(gmes_sim* s, int32_t c, int64_t cell, int32_t* id) {
   return GMES_OK;
 }
 
-static std::vector<int64_t>& probe_cursor(gmes_sim* s) {
-  static thread_local std::vector<std::pair<gmes_sim*, std::vector<int64_t>>> table;
-  for (auto& t : table)
-    if (t.first == s) return t.second;
-  table.push_back({s, {}});
-  return table.back().second;
-}
-
 int gmes_sim_read_probe(gmes_sim* s, int32_t id, double* out, int64_t max, int64_t* n) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   if (id < 0 || id >= (int32_t)s->probes.size()) return set_error(GMES_EINVAL, "bad probe id");
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(sync_exec(s));
-  std::vector<int64_t>& cur = probe_cursor(s);
+  std::vector<int64_t>& cur = s->probe_read;
   cur.resize(s->probes.size(), 0);
   const int64_t avail = s->probe_n - cur[id];
   const int64_t m = std::min(avail, max);
@@ -1252,6 +1272,7 @@ int gmes_sim_read_probe(gmes_sim* s, int32_t id, double* out, int64_t max, int64
 int gmes_sim_add_dft(gmes_sim* s, int32_t c, const int64_t lo[3], const int64_t hi[3], int32_t nfreq,
                      int32_t decim, int32_t* id) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   if (c < 0 || c > 5 || !active(s, c)) return set_error(GMES_EINVAL, "DFT of an inactive component");
   if (nfreq < 1 || decim < 1) return set_error(GMES_EINVAL, "nfreq and decim must be positive");
   if (!s->dfts.empty() && nfreq != s->dft_nfreq)
@@ -1282,6 +1303,7 @@ int gmes_sim_add_dft(gmes_sim* s, int32_t c, const int64_t lo[3], const int64_t 
 
 int gmes_sim_set_dft_weights(gmes_sim* s, int64_t step0, int64_t nsteps, const double* w) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   if (s->dfts.empty()) return set_error(GMES_EINVAL, "no DFT monitor");
   if (nsteps < 0) return set_error(GMES_EINVAL, "negative step count");
   HIP_TRY(hipSetDevice(s->device));
@@ -1304,6 +1326,7 @@ int gmes_sim_set_dft_weights(gmes_sim* s, int64_t step0, int64_t nsteps, const d
 
 int gmes_sim_read_dft(gmes_sim* s, int32_t id, double* out, int64_t count) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   if (id < 0 || id >= (int32_t)s->dfts.size()) return set_error(GMES_EINVAL, "bad DFT id");
   const auto& d = s->dfts[id];
   if (count != (int64_t)d.nfreq * d.ncells * 2) return set_error(GMES_EINVAL, "size mismatch");
@@ -1315,6 +1338,7 @@ int gmes_sim_read_dft(gmes_sim* s, int32_t id, double* out, int64_t count) {
 
 int gmes_sim_write_field(gmes_sim* s, int32_t c, const void* host, int64_t bytes) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   if (c < 0 || c > 5) return set_error(GMES_EINVAL, "bad component");
   if (bytes != s->ncell[c] * s->esize) return set_error(GMES_EINVAL, "size mismatch");
   HIP_TRY(hipSetDevice(s->device));
@@ -1334,6 +1358,7 @@ int gmes_sim_write_field(gmes_sim* s, int32_t c, const void* host, int64_t bytes
 
 int gmes_sim_read_field(gmes_sim* s, int32_t c, void* host, int64_t bytes) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   if (c < 0 || c > 5) return set_error(GMES_EINVAL, "bad component");
   if (bytes != s->ncell[c] * s->esize) return set_error(GMES_EINVAL, "size mismatch");
   HIP_TRY(hipSetDevice(s->device));
@@ -1364,6 +1389,7 @@ static void state_blocks(gmes_sim* s, std::vector<std::pair<void*, size_t>>& out
 
 int gmes_sim_state_bytes(const gmes_sim* s, int64_t* bytes) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   std::vector<std::pair<void*, size_t>> b;
   state_blocks(const_cast<gmes_sim*>(s), b);
   int64_t t = 0;
@@ -1374,6 +1400,7 @@ int gmes_sim_state_bytes(const gmes_sim* s, int64_t* bytes) {
 
 int gmes_sim_snapshot(gmes_sim* s, void* host, int64_t bytes) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   std::vector<std::pair<void*, size_t>> b;
   state_blocks(s, b);
   int64_t t = 0;
@@ -1388,6 +1415,7 @@ int gmes_sim_snapshot(gmes_sim* s, void* host, int64_t bytes) {
 
 int gmes_sim_restore(gmes_sim* s, const void* host, int64_t bytes) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   std::vector<std::pair<void*, size_t>> b;
   state_blocks(s, b);
   int64_t t = 0;
@@ -1402,11 +1430,13 @@ int gmes_sim_restore(gmes_sim* s, const void* host, int64_t bytes) {
 
 int gmes_sim_step(gmes_sim* s, int64_t nsteps) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   return do_step(s, nsteps);
 }
 
 int gmes_sim_sync(gmes_sim* s) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(sync_exec(s));
   Ctl c;
@@ -1414,6 +1444,7 @@ int gmes_sim_sync(gmes_sim* s) {
   if (c.err == 1) return set_error(GMES_ERUNTIME, "Dm2 corrector produced an invalid error");
   if (c.err == 2) return set_error(GMES_ERUNTIME, "Dm2 corrector failed to converge");
   for (int i = 0; i < s->naux; ++i) {
+    if (!s->aux[i]) continue;
     int rc = gmes_sim_sync(s->aux[i]);
     if (rc) return rc;
   }
@@ -1423,12 +1454,14 @@ int gmes_sim_sync(gmes_sim* s) {
 
 int gmes_sim_get_step(const gmes_sim* s, int64_t* step) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   *step = s->step;
   return GMES_OK;
 }
 
 int gmes_sim_set_graph(gmes_sim* s, int32_t enable) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   s->use_graph = enable != 0;
   destroy_graph(s);
   return GMES_OK;
@@ -1436,6 +1469,7 @@ int gmes_sim_set_graph(gmes_sim* s, int32_t enable) {
 
 int gmes_sim_set_ghost_fusion(gmes_sim* s, int32_t enable, int32_t* fused) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(sync_exec(s));
   s->allow_fuse = enable != 0;
@@ -1446,6 +1480,7 @@ int gmes_sim_set_ghost_fusion(gmes_sim* s, int32_t enable, int32_t* fused) {
 
 int gmes_sim_set_tile(gmes_sim* s, int32_t bx, int32_t by) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   if (bx < 0 || by < 0 || bx * by > 256 || (bx && (bx & (bx - 1))))
     return set_error(GMES_EINVAL, "bad tile: bx must be a power of two and bx*by <= 256");
   s->tile_bx = bx;
@@ -1458,6 +1493,7 @@ int gmes_sim_set_tile(gmes_sim* s, int32_t bx, int32_t by) {
 
 int gmes_sim_autotune(gmes_sim* s, int32_t* bx, int32_t* by) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   if (!s->committed) return set_error(GMES_EINVAL, "sim is not committed");
   HIP_TRY(hipSetDevice(s->device));
   int tbx = 0, tby = 0, rc;
@@ -1480,6 +1516,7 @@ int gmes_sim_autotune(gmes_sim* s, int32_t* bx, int32_t* by) {
 
 int gmes_sim_profile(gmes_sim* s, int32_t enable) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(sync_exec(s));
   collect_timing(s);
@@ -1490,6 +1527,7 @@ int gmes_sim_profile(gmes_sim* s, int32_t enable) {
 
 int gmes_sim_read_profile(gmes_sim* s, double ms[6], int64_t launches[6]) {
   ApiGuard api;
+  if (!s) return set_error(GMES_EINVAL, "null sim handle");
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(sync_exec(s));
   int rc = collect_timing(s);
@@ -1714,6 +1752,7 @@ int gmes_batch_destroy(gmes_batch* b) {
 
 int gmes_batch_set_graph(gmes_batch* b, int32_t enable) {
   ApiGuard api;
+  if (!b) return set_error(GMES_EINVAL, "null batch handle");
   b->use_graph = enable != 0;
   destroy_batch_graph(b);
   return GMES_OK;
@@ -1723,9 +1762,11 @@ int gmes_batch_set_graph(gmes_batch* b, int32_t enable) {
 // half step, replayed from one captured graph per step.
 int gmes_batch_step(gmes_batch* b, int64_t nsteps) {
   ApiGuard api;
+  if (!b) return set_error(GMES_EINVAL, "null batch handle");
   if (nsteps < 0) return set_error(GMES_EINVAL, "negative step count");
   for (gmes_sim* s : b->sims) {
     if (s->timing.on) return set_error(GMES_EINVAL, "kernel timing is per sim; switch it off for batch members");
+    if (s->aux_lost) return set_error(GMES_EINVAL, "an auxiliary grid of a batch member was destroyed");
     int rc = check_windows(s, nsteps);
     if (rc) return rc;
     if ((rc = ensure_probe_capacity(s, nsteps))) return rc;
@@ -1768,7 +1809,8 @@ int gmes_batch_step(gmes_batch* b, int64_t nsteps) {
   std::vector<uint64_t> gens;
   for (gmes_sim* s : b->sims) {
     gens.push_back(s->gen);
-    for (int i = 0; i < s->naux; ++i) gens.push_back(s->aux[i]->gen);
+    for (int i = 0; i < s->naux; ++i)
+      if (s->aux[i]) gens.push_back(s->aux[i]->gen);
   }
   if (sig != b->shape_sig || gens != b->gens) destroy_batch_graph(b);
   b->shape_sig = sig;
@@ -1797,6 +1839,7 @@ int gmes_batch_step(gmes_batch* b, int64_t nsteps) {
 
 int gmes_batch_sync(gmes_batch* b) {
   ApiGuard api;
+  if (!b) return set_error(GMES_EINVAL, "null batch handle");
   HIP_TRY(hipSetDevice(b->device));
   HIP_TRY(hipStreamSynchronize(b->stream));
   for (gmes_sim* s : b->sims) {

@@ -115,6 +115,7 @@ struct gmes_sim {
   // ade_on_ghost_source: a dispersive E cell lies on plane 0 of a
   // transverse axis (set by commit), which rules that out
   bool fuse_e = false, ade_on_ghost_source = false, allow_fuse = true;
+  bool aux_lost = false;  // an attached auxiliary grid was destroyed
   uint64_t gen = 0;            // bumped whenever captured launch arguments go stale
   void* d_aux_ptrs = nullptr;  // [8][6] device pointers of aux fields
   // probes
@@ -124,6 +125,7 @@ struct gmes_sim {
   int32_t* d_probe_comps = nullptr;
   double* d_probe_buf = nullptr;   // [nprobe][cap][2]
   int64_t probe_cap = 0, probe_base = 0, probe_n = 0;  // recorded steps since base
+  std::vector<int64_t> probe_read;  // per probe: values read since probe_base
   // running-DFT monitors: boxes of storage indices, FP64 accumulators
   struct Dft {
     int32_t comp;

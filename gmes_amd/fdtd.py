@@ -397,7 +397,8 @@ class FDTD(object):
             self._lib.set_dft_weights(self._sim, step0,
                                       dft_weights(self._dft_freqs, step0, nsteps, self.time_step.dt))
         for aux in self._aux_fdtds:
-            aux._prepare(nsteps)
+            if aux._sim is not None:  # a closed grid: the engine refuses the step
+                aux._prepare(nsteps)
 
     def _advance(self, nsteps):
         """Run nsteps whole steps on the device and update the bookkeeping."""
@@ -416,7 +417,8 @@ class FDTD(object):
         for c in const.COMPONENTS:
             self._dev_newer[c] = True
         for aux in self._aux_fdtds:
-            aux._after_steps(nsteps)
+            if aux._sim is not None:
+                aux._after_steps(nsteps)
         if self.e_recorder or self.h_recorder:
             self._lib.sync(self._sim)
             n_last = self.time_step.n

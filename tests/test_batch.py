@@ -109,3 +109,25 @@ def test_autotune_keeps_results_and_sim_state(engine_lib):
     fa, fb = fields(a), fields(b)
     for c in fa:
         np.testing.assert_array_equal(fa[c], fb[c])
+
+
+@pytest.mark.gpu
+def test_member_and_aux_lifetimes_are_enforced(engine_lib):
+    """A batch member cannot be destroyed while its batch exists, and a sim
+    whose auxiliary TFSF grid was destroyed refuses to step (its source
+    operations read that grid)."""
+    from cases import kat_tmz, tfsf2d
+    a, b = kat_tmz(engine_lib), kat_tmz(engine_lib)
+    bt = engine_lib.batch_create([a._sim, b._sim])
+    with pytest.raises(ValueError):
+        engine_lib.sim_destroy(a._sim)
+    engine_lib.batch_destroy(bt)
+    a._advance(3)
+    a.close()
+    b.close()
+    sim = tfsf2d(engine_lib, res=10)
+    sim._advance(2)
+    sim._aux_fdtds[0].close()
+    with pytest.raises(ValueError):
+        sim._advance(1)
+    sim.close()
