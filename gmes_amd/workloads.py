@@ -54,11 +54,11 @@ def spheres3d_geometry(n=256, nsph=40, pml=12, seed=2):
     return geom
 
 
-def spheres3d(n=256, precision="fp32", nsph=40, pml=None, seed=2, _engine=None, verbose=False):
+def spheres3d(n=256, precision="fp32", nsph=40, pml=None, seed=2, _engine=None, verbose=False, amp=1):
     pml = 12 * n / 256.0 if pml is None else pml
     space = _geo.Cartesian(size=(n, n, n), resolution=1)
     geom = spheres3d_geometry(n, nsph, pml, seed)
-    src = [_src.PointSource(_src.Bandpass(freq=0.05, fwidth=0.025), center=(0, 0, 0), component=Ex)]
+    src = [_src.PointSource(_src.Bandpass(freq=0.05, fwidth=0.025), center=(0, 0, 0), component=Ex, amp=amp)]
     sim = _fdtd.FDTD(space, geom, src, courant_ratio=0.5, verbose=verbose, precision=precision,
                      _engine=_engine)
     sim.init()
@@ -229,7 +229,7 @@ def serve_request_list(n=48, rate=20.0, seed=5, nfreq=50, decimation=4, precisio
     return out
 
 
-def dispersive3d(scale=1.0, seed=3, precision="fp32", _engine=None, courant=0.3, src_time=None):
+def dispersive3d(scale=1.0, seed=3, precision="fp32", _engine=None, courant=0.3, src_time=None, amp=1):
     """3-D dispersive Bloch cell: 384 x 384 x 192 cells (times ``scale``) at
     dx = 1; a periodic 8 x 8 array of Drude blocks (24 cells, pitch 48;
     wp in [1, 10], gamma in [0.01, 0.1]), 16 seed-placed Lorentz spheres
@@ -273,7 +273,7 @@ def dispersive3d(scale=1.0, seed=3, precision="fp32", _engine=None, courant=0.3,
     st = _src.Bandpass(freq=0.05, fwidth=0.025) if src_time is None else src_time
     src = [_src.TotalFieldScatteredField(st, center=(0, 0, 0.25),
                                          size=(2 * nx, 2 * ny, 2 * ztop - 0.5), direction=(0, 0, -1),
-                                         polarization=(1, 0, 0))]
+                                         polarization=(1, 0, 0), amp=amp)]
     sim = _fdtd.FDTD(space, geom, src, courant_ratio=courant, bloch=(kx, ky, 0.0), verbose=False,
                      precision=precision, _engine=_engine)
     sim.init()

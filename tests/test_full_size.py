@@ -1,0 +1,88 @@
+"""Full-size runs of the benchmark workloads checked through properties that
+do not depend on size: the update is linear in the source amplitude, and
+scaling every input by 2 scales every rounded result by exactly 2 while the
+values stay normal, so doubling the amplitude must double every field word
+bit for bit (FP64: no value there comes near the subnormal range).  In
+FP32 the leading edge of a wavefront passes through subnormal values, where
+rounding is not scale invariant, and the wave carries those differences
+along as it grows; there the check is linearity to 1e-6 of the peak in
+every word and 1e-6 in relative L2."""
+
+import numpy as np
+import pytest
+
+import gmes_amd as g
+from gmes_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+def _fields(sim):
+    return {c: np.asarray(sim.field[c]).copy() for c in sim._active}
+
+
+def _assert_doubled(f1, f2, exact=True):
+    nonzero = 0
+    peak = max(float(np.abs(a).max()) for a in f1.values())
+    num = den = 0.0
+    for c in f1:
+        a, b = f1[c], f2[c]
+        if exact:
+            np.testing.assert_array_equal(b, 2 * a)
+        else:
+            d = np.abs(b.astype(np.complex128) - 2 * a.astype(np.complex128))
+            assert float(d.max()) <= 1e-6 * 2 * peak
+            num += float(np.sum(d ** 2))
+            den += float(np.sum(np.abs(2 * a.astype(np.complex128)) ** 2))
+        nonzero += int(np.count_nonzero(a))
+    assert nonzero > 0 and peak > 0
+    if not exact:
+        assert (num / den) ** 0.5 <= 1e-6
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_spheres256_linear_in_amplitude(precision):
+    """The 256^3 spheres + CPML workload at full size (FP64: 120 steps,
+    FP32: 800 steps)."""
+    out = []
+    for amp in (1, 2):
+        sim = workloads.spheres3d(256, precision=precision, amp=amp)
+        sim._advance(120 if precision == "fp64" else 800)
+        out.append(_fields(sim))
+        sim.close()
+    if precision == "fp64":
+        smallest = min(np.abs(v[v != 0]).min() for v in out[0].values() if np.any(v))
+        assert smallest > 1e-290  # no subnormal values: exact scaling applies
+        _assert_doubled(*out)
+    else:
+        _assert_doubled(*out, exact=False)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_dispersive_cell_linear_in_amplitude(precision):
+    """The 384 x 384 x 192 dispersive Bloch cell (Drude, Lorentz, DCP, CPML,
+    TFSF with its auxiliary grid) at full size, complex (FP64: 60 steps,
+    FP32: 300 steps)."""
+    out = []
+    st = g.Bandpass(freq=0.05, fwidth=0.1, s=3)
+    for amp in (1, 2):
+        sim = workloads.dispersive3d(precision=precision, amp=amp, src_time=st)
+        sim._advance(60 if precision == "fp64" else 300)
+        out.append(_fields(sim))
+        sim.close()
+    if precision == "fp64":
+        _assert_doubled(*out)
+    else:
+        _assert_doubled(*out, exact=False)
+
+
+def test_spheres256_fp32_tracks_fp64():
+    """FP32 against FP64 on the full 256^3 workload after 300 steps."""
+    f = {}
+    for p in ("fp32", "fp64"):
+        sim = workloads.spheres3d(256, precision=p)
+        sim._advance(300)
+        f[p] = _fields(sim)
+        sim.close()
+    num = sum(float(np.sum((f["fp32"][c].astype(np.float64) - f["fp64"][c]) ** 2)) for c in f["fp64"])
+    den = sum(float(np.sum(f["fp64"][c] ** 2)) for c in f["fp64"])
+    assert den > 0 and (num / den) ** 0.5 <= 1e-5
