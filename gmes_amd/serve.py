@@ -3,8 +3,11 @@
 Requests arrive over time.  Each is a callable that builds an initialised
 simulation and a result monitor, plus a step count.  The loop:
 
-* admits queued requests in arrival order while the sum of their estimated
-  HBM footprints fits the budget and fewer than ``max_active`` run;
+* admits queued requests while the sum of their estimated HBM footprints
+  fits the budget and fewer than ``max_active`` run: in arrival order, or
+  with ``policy="shortest"`` the queued request of least estimated work
+  first (lower median latency when many wait; a request that does not fit
+  is passed over only while a smaller one does);
 * builds admitted requests on worker threads (host tables, device
   allocation and uploads), so setup overlaps with the GPU running others;
 * advances every active simulation together, as one batch: the main,
@@ -31,15 +34,18 @@ from .batch import Batch
 
 
 class Request(object):
-    def __init__(self, rid, build, steps, arrival=0.0, footprint=0):
+    def __init__(self, rid, build, steps, arrival=0.0, footprint=0, work=None):
         """build() -> (sim, result_fn): an initialised FDTD and a function
         returning the request's result once it has run ``steps`` steps.
-        footprint: estimated device bytes, used for admission."""
+        footprint: estimated device bytes, used for admission; work: an
+        estimate of its cost (cell-updates), used by the "shortest" policy
+        (default footprint x steps)."""
         self.rid = rid
         self.build = build
         self.steps = int(steps)
         self.arrival = float(arrival)
         self.footprint = int(footprint)
+        self.work = float(work) if work is not None else float(self.footprint) * self.steps
         self.sim = None
         self.result_fn = None
         self.result = None
@@ -53,7 +59,11 @@ class Request(object):
 
 
 class Server(object):
-    def __init__(self, hbm_budget, max_active=32, chunk=250, workers=4, autotune=True, clock=None):
+    def __init__(self, hbm_budget, max_active=32, chunk=250, workers=4, autotune=True, clock=None,
+                 policy="arrival"):
+        if policy not in ("arrival", "shortest"):
+            raise ValueError("policy must be 'arrival' or 'shortest'")
+        self.policy = policy
         self.hbm_budget = int(hbm_budget)
         self.max_active = int(max_active)
         self.chunk = int(chunk)
@@ -83,20 +93,25 @@ class Server(object):
             while pending or queue or building or active:
                 t = now()
                 while pending and pending[0].arrival <= t:
-                    queue.append(pending.pop(0))
-                # admission: arrival order, by estimated footprint
-                while (queue and len(active) + len(building) < self.max_active
-                       and reserved + queue[0].footprint <= self.hbm_budget):
-                    r = queue.pop(0)
+                    r = pending.pop(0)
+                    if r.footprint > self.hbm_budget:  # can never be admitted
+                        r.error = "footprint %d exceeds the HBM budget" % r.footprint
+                        r.t_done = now()
+                        done.append(r)
+                    else:
+                        queue.append(r)
+                # admission by estimated footprint, in policy order
+                if self.policy == "shortest":
+                    queue.sort(key=lambda q: (q.work, q.arrival))
+                while queue and len(active) + len(building) < self.max_active:
+                    fits = [q for q in queue if reserved + q.footprint <= self.hbm_budget]
+                    if not fits or (self.policy == "arrival" and fits[0] is not queue[0]):
+                        break
+                    r = fits[0]
+                    queue.remove(r)
                     reserved += r.footprint
                     r.t_admit = now()
                     building[r] = pool.submit(r.build)
-                if queue and not active and not building and queue[0].footprint > self.hbm_budget:
-                    r = queue.pop(0)
-                    r.error = "footprint %d exceeds the HBM budget" % r.footprint
-                    r.t_done = now()
-                    done.append(r)
-                    continue
                 self.peak_reserved = max(self.peak_reserved, reserved)
                 for r, fut in list(building.items()):
                     if fut.done():

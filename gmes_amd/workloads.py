@@ -223,8 +223,9 @@ def serve_request_list(n=48, rate=20.0, seed=5, nfreq=50, decimation=4, precisio
 
     out = []
     for i, (t, side, radius, nsteps) in enumerate(serve_requests(n, rate, seed, lo, hi)):
-        out.append(Request(i, builder(side, radius), nsteps if steps is None else steps, arrival=t,
-                           footprint=pc_slab_footprint(side, nfreq)))
+        n = nsteps if steps is None else steps
+        out.append(Request(i, builder(side, radius), n, arrival=t, footprint=pc_slab_footprint(side, nfreq),
+                           work=float(side) ** 3 * n))
         out[-1].side = side
     return out
 

@@ -109,3 +109,24 @@ def test_tables_built_in_another_process_are_identical(oracle_lib):
     for c in g.constant.COMPONENTS:
         np.testing.assert_array_equal(np.asarray(a.field[c]), np.asarray(b.field[c]))
     np.testing.assert_array_equal(fa.flux(), fb.flux())
+
+
+def test_shortest_policy_admits_least_work_first(oracle_lib):
+    """With one slot and every request waiting at the start, the "shortest"
+    policy serves them in order of estimated work and the "arrival" policy
+    in arrival order; results do not depend on the order."""
+    ref = alone(oracle_lib)
+    order = {}
+    for policy in ("arrival", "shortest"):
+        reqs = requests(oracle_lib)
+        for r in reqs:
+            r.arrival = 0.0
+        done = Server(hbm_budget=10 ** 9, max_active=1, chunk=16, workers=1, policy=policy).run(reqs)
+        order[policy] = [r.rid for r in sorted(done, key=lambda r: r.t_done)]
+        for r in done:
+            np.testing.assert_array_equal(r.result, ref[r.rid])
+    steps = {r.rid: r.steps for r in requests(oracle_lib)}
+    assert order["arrival"] == list(range(5))
+    assert order["shortest"] == sorted(range(5), key=lambda i: steps[i])
+    with pytest.raises(ValueError):
+        Server(hbm_budget=1, policy="fifo")
