@@ -196,17 +196,27 @@ __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typ
     if (cls == GMES_CLS_CPML) {
       const GMES_GLOBAL float* p1 = (const GMES_GLOBAL float*)m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 3;
       const GMES_GLOBAL float* p2 = (const GMES_GLOBAL float*)m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 3;
-      // real FP32: the branches around the psi words cost more than their
-      // traffic (measured), so every CPML cell moves both
-      constexpr bool kElide = sizeof(T) >= 8;
-      const bool u1 = !kElide || psi_live<Ax<C>::a1>(rec, ix);
-      const bool u2 = !kElide || psi_live<Ax<C>::a2>(rec, ix);
-      const T o1 = u1 ? gld(st + flat) : vzero<T>();
-      const T o2 = u2 ? gld(st + m.dsize[C] + flat) : vzero<T>();
-      const T psi1 = fma_(p1[1], a, vscl(p1[0], o1));
-      const T psi2 = fma_(p2[1], b, vscl(p2[0], o2));
-      if (u1) gst(st + flat, psi1);
-      if (u2) gst(st + m.dsize[C] + flat, psi2);
+      const bool u1 = psi_live<Ax<C>::a1>(rec, ix), u2 = psi_live<Ax<C>::a2>(rec, ix);
+      T psi1, psi2;
+      if constexpr (sizeof(T) >= 8) {
+        const T o1 = u1 ? gld(st + flat) : vzero<T>();
+        const T o2 = u2 ? gld(st + m.dsize[C] + flat) : vzero<T>();
+        psi1 = fma_(p1[1], a, vscl(p1[0], o1));
+        psi2 = fma_(p2[1], b, vscl(p2[0], o2));
+        if (u1) gst(st + flat, psi1);
+        if (u2) gst(st + m.dsize[C] + flat, psi2);
+      } else {
+        // real FP32: branches around the psi words cost more than their
+        // traffic (measured).  A dead psi is read from element 0 of its
+        // array instead, a guard word that is always +0 (one cache line
+        // for all such lanes), and its +0 is stored back unconditionally.
+        const T o1 = gld(st + (u1 ? flat : 0));
+        const T o2 = gld(st + m.dsize[C] + (u2 ? flat : 0));
+        psi1 = fma_(p1[1], a, vscl(p1[0], o1));
+        psi2 = fma_(p2[1], b, vscl(p2[0], o2));
+        gst(st + flat, psi1);
+        gst(st + m.dsize[C] + flat, psi2);
+      }
       const T t = vsub(vadd(fma_(-p2[2], b, vscl(p1[2], a)), psi1), psi2);
       return E ? fma_(rec.c0, t, f) : fma_(-rec.c0, t, f);
     }
