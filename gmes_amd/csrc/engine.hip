@@ -140,6 +140,34 @@ void free_ade(gmes_sim* s) {
     }
 }
 
+// Byte copies of the material ids for the real-FP32 3-D main kernels,
+// which then read 1 byte per cell and component instead of 2, when the
+// record table has at most 256 entries (other sims and kernels read the
+// 16-bit ids; for wider words the saving measured below the cost).
+int upload_byte_ids(gmes_sim* s) {
+  const bool narrow = s->g.active == 0x3Fu && s->g.dtype == GMES_F32 && s->recs.size() <= 256;
+  for (int c = 0; c < 6; ++c) {
+    if (!narrow || !active(s, c)) {
+      if (s->ids8[c]) { (void)dev_release(s->ids8[c]); s->ids8[c] = nullptr; }
+      continue;
+    }
+    if (!s->ids8[c]) {
+      int rc = dev_alloc(s, (void**)&s->ids8[c], (size_t)s->dsize[c]);
+      if (rc) return rc;
+    }
+    const std::vector<uint16_t>& ids = s->host_ids[c];
+    std::vector<uint8_t> dev((size_t)s->dsize[c], 0);
+    const int64_t rows = s->shape[c][0] * s->shape[c][1], n2 = s->shape[c][2];
+    for (int64_t r = 0; r < rows; ++r) {
+      uint8_t* d = dev.data() + kPad - s->koff[c] + r * s->pitch[c];
+      const uint16_t* h = ids.data() + r * n2;
+      for (int64_t k = 0; k < n2; ++k) d[k] = (uint8_t)h[k];
+    }
+    HIP_TRY(hipMemcpy(s->ids8[c], dev.data(), dev.size(), hipMemcpyHostToDevice));
+  }
+  return GMES_OK;
+}
+
 // Coordinates along `axis` where the CPML c coefficient of profile set p is
 // exactly zero for every active component: the longest such run [lo, hi),
 // packed as lo | hi << 16 (0 = empty: every coordinate keeps its psi).
@@ -345,8 +373,11 @@ bool main_args(const gmes_sim* s, bool electric, MainArgs<T>& m, int ext[3]) {
     for (int d = 0; d < 3; ++d) ext[d] = std::max<int>(ext[d], (int)s->shape[c][d]);
   }
   if (!any) return false;
+  m.narrow = 1;
   for (int c = 0; c < 6; ++c) {
     m.ids[c] = s->ids[c];
+    m.ids8[c] = s->ids8[c];
+    if (active(s, c) && !s->ids8[c]) m.narrow = 0;
     m.pml[c] = s->pml_state[c];
     m.dsize[c] = (int)s->dsize[c];
   }
@@ -935,6 +966,7 @@ int gmes_sim_destroy(gmes_sim* s) {
   for (int c = 0; c < 6; ++c) {
     if (s->f[c]) (void)dev_release(s->f[c]);
     if (s->ids[c]) (void)dev_release(s->ids[c]);
+    if (s->ids8[c]) (void)dev_release(s->ids8[c]);
     if (s->pml_state[c]) (void)dev_release(s->pml_state[c]);
   }
   free_ade(s);
@@ -1135,6 +1167,7 @@ int gmes_sim_commit(gmes_sim* s) {
   if (s->d_prof_off) { (void)dev_release(s->d_prof_off); s->d_prof_off = nullptr; }
   int rc = s->fp64 ? upload_tables<double>(s) : upload_tables<float>(s);
   if (rc) return rc;
+  if ((rc = upload_byte_ids(s))) return rc;
   s->committed = true;
   destroy_graph(s);
   decide_fusion(s);

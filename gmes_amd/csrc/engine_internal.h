@@ -82,6 +82,7 @@ struct gmes_sim {
   int64_t dsize[6];          // device elements per component
   void* f[6] = {};           // device fields
   uint16_t* ids[6] = {};     // device ids
+  uint8_t* ids8[6] = {};     // byte copies for the 3-D main kernels (at most 256 records)
   std::vector<uint16_t> host_ids[6];
   std::vector<gmes_mat_rec> recs;
   std::vector<double> coef;

@@ -61,6 +61,8 @@ struct MainArgs {
   int fuse;
   int gax[3][2];      // transverse axes a1, a2 of Ex, Ey, Ez
   cplx<R> gph[3][2];  // ghost phase along a1, a2
+  const uint8_t* ids8[6];  // the ids as bytes (real-FP32 3-D sims), when narrow
+  int narrow;
 };
 
 template <class T>

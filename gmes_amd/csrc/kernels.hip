@@ -384,7 +384,9 @@ __device__ __forceinline__ void load_e(Planes<T, N>& q, int i0, int j, int k, co
                                        const int* W, const T* __restrict__ hx, const T* __restrict__ hy,
                                        const T* __restrict__ hz, const T* __restrict__ ex, const T* __restrict__ ey,
                                        const T* __restrict__ ez, const uint16_t* __restrict__ idx,
-                                       const uint16_t* __restrict__ idy, const uint16_t* __restrict__ idz) {
+                                       const uint16_t* __restrict__ idy, const uint16_t* __restrict__ idz, const uint8_t* __restrict__ bx8,
+                                       const uint8_t* __restrict__ by8, const uint8_t* __restrict__ bz8,
+                                       bool narrow) {
 #pragma unroll
   for (int n = 0; n < N; ++n) {
     const int i = i0 + n;
@@ -400,9 +402,15 @@ __device__ __forceinline__ void load_e(Planes<T, N>& q, int i0, int j, int k, co
     q.o[0][n] = ld_nt(ex + fx);
     q.o[1][n] = ld_nt(ey + fy);
     q.o[2][n] = ld_nt(ez + fz);
-    q.r[0][n] = ld_nt(idx + fx);
-    q.r[1][n] = ld_nt(idy + fy);
-    q.r[2][n] = ld_nt(idz + fz);
+    if (sizeof(T) == 4 && narrow) {  // byte ids: real FP32 only (measured)
+      q.r[0][n] = ld_nt(bx8 + fx);
+      q.r[1][n] = ld_nt(by8 + fy);
+      q.r[2][n] = ld_nt(bz8 + fz);
+    } else {
+      q.r[0][n] = ld_nt(idx + fx);
+      q.r[1][n] = ld_nt(idy + fy);
+      q.r[2][n] = ld_nt(idz + fz);
+    }
   }
 }
 
@@ -441,7 +449,9 @@ __device__ __forceinline__ void load_h(Planes<T, N>& q, int i0, int j, int k, co
                                        const int* W, const T* __restrict__ ex, const T* __restrict__ ey,
                                        const T* __restrict__ ez, const T* __restrict__ hx, const T* __restrict__ hy,
                                        const T* __restrict__ hz, const uint16_t* __restrict__ idx,
-                                       const uint16_t* __restrict__ idy, const uint16_t* __restrict__ idz) {
+                                       const uint16_t* __restrict__ idy, const uint16_t* __restrict__ idz, const uint8_t* __restrict__ bx8,
+                                       const uint8_t* __restrict__ by8, const uint8_t* __restrict__ bz8,
+                                       bool narrow) {
 #pragma unroll
   for (int n = 0; n < N; ++n) {
     const int i = i0 + n;
@@ -458,9 +468,15 @@ __device__ __forceinline__ void load_h(Planes<T, N>& q, int i0, int j, int k, co
     q.o[0][n] = ld_nt(hx + fx);
     q.o[1][n] = ld_nt(hy + fy);
     q.o[2][n] = ld_nt(hz + fz);
-    q.r[0][n] = ld_nt(idx + fx);
-    q.r[1][n] = ld_nt(idy + fy);
-    q.r[2][n] = ld_nt(idz + fz);
+    if (sizeof(T) == 4 && narrow) {  // byte ids: real FP32 only (measured)
+      q.r[0][n] = ld_nt(bx8 + fx);
+      q.r[1][n] = ld_nt(by8 + fy);
+      q.r[2][n] = ld_nt(bz8 + fz);
+    } else {
+      q.r[0][n] = ld_nt(idx + fx);
+      q.r[1][n] = ld_nt(idy + fy);
+      q.r[2][n] = ld_nt(idz + fz);
+    }
   }
 }
 
@@ -532,6 +548,9 @@ __device__ __forceinline__ void main3d_tile(const MainArgs<T>& m, DevRec<typenam
   const uint16_t* ida = m.ids[E ? 0 : 3];
   const uint16_t* idb = m.ids[E ? 1 : 4];
   const uint16_t* idc = m.ids[E ? 2 : 5];
+  const uint8_t* i8a = m.ids8[E ? 0 : 3];
+  const uint8_t* i8b = m.ids8[E ? 1 : 4];
+  const uint8_t* i8c = m.ids8[E ? 2 : 5];
   const DevRec<R>* grec = m.recs;
   RecStage<R> stage;
   stage.load(grec, m.nrec, tid, nthreads);
@@ -540,11 +559,11 @@ __device__ __forceinline__ void main3d_tile(const MainArgs<T>& m, DevRec<typenam
     T u0 = T{}, u1 = T{};
     if (act) {
       if constexpr (E) {
-        load_e<T, N>(q, i0, j, k, B, P, W, f[3], f[4], f[5], f[0], f[1], f[2], ida, idb, idc);
+        load_e<T, N>(q, i0, j, k, B, P, W, f[3], f[4], f[5], f[0], f[1], f[2], ida, idb, idc, i8a, i8b, i8c, m.narrow != 0);
         u0 = gld(f[5] + B[5] + i0 * P[5] + (j + 1) * W[5] + k);
         u1 = gld(f[4] + B[4] + i0 * P[4] + j * W[4] + k + 1);
       } else {
-        load_h<T, N>(q, i0, j, k, B, P, W, f[0], f[1], f[2], f[3], f[4], f[5], ida, idb, idc);
+        load_h<T, N>(q, i0, j, k, B, P, W, f[0], f[1], f[2], f[3], f[4], f[5], ida, idb, idc, i8a, i8b, i8c, m.narrow != 0);
         u0 = gld(f[1] + B[1] + i0 * P[1] + j * W[1] + k);
         u1 = gld(f[2] + B[2] + i0 * P[2] + j * W[2] + k);
       }
@@ -577,10 +596,10 @@ __device__ __forceinline__ void main3d_tile(const MainArgs<T>& m, DevRec<typenam
     for (int i = i0; i < nx; ++i) {
       Planes<T, 1> q;
       if constexpr (E) {
-        load_e<T, 1>(q, i, j, k, B, P, W, f[3], f[4], f[5], f[0], f[1], f[2], ida, idb, idc);
+        load_e<T, 1>(q, i, j, k, B, P, W, f[3], f[4], f[5], f[0], f[1], f[2], ida, idb, idc, i8a, i8b, i8c, m.narrow != 0);
         update_e<T, 1>(m, recs, q, i, j, k, B, P, W, f[0], f[1], f[2], sx, sy, sz, u0, u1);
       } else {
-        load_h<T, 1>(q, i, j, k, B, P, W, f[0], f[1], f[2], f[3], f[4], f[5], ida, idb, idc);
+        load_h<T, 1>(q, i, j, k, B, P, W, f[0], f[1], f[2], f[3], f[4], f[5], ida, idb, idc, i8a, i8b, i8c, m.narrow != 0);
         update_h<T, 1>(m, recs, q, i, j, k, B, P, W, f[3], f[4], f[5], sx, sy, sz, u0, u1);
       }
     }
