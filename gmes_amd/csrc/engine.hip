@@ -85,6 +85,20 @@ const int kAx2[6] = {2, 0, 1, 2, 0, 1};
 const int kGhostOrder[6][2] = {{1, 2}, {2, 0}, {0, 1}, {1, 2}, {2, 0}, {0, 1}};
 
 bool active(const gmes_sim* s, int c) { return (s->g.active >> c) & 1u; }
+bool is3d(const gmes_sim* s) { return s->g.active == 0x3Fu; }
+// A 2-D mode with a collapsed z axis (TMz, TEz): every active component
+// has rows of at most 2 elements and exactly one updated z plane (its own
+// axis z has extent 1; a z extent of 2 holds one ghost plane).  Single-sim
+// steps of such modes use k_main2z.
+bool is_2z(const gmes_sim* s) {
+  if (is3d(s)) return false;
+  for (int c = 0; c < 6; ++c) {
+    if (!active(s, c)) continue;
+    const int64_t* sh = s->shape[c];
+    if (sh[2] > 2 || (c % 3 == 2 && sh[2] != 1) || sh[0] < 2 || sh[1] < 2) return false;
+  }
+  return true;
+}
 
 // device element of logical (i,j,k) / of a logical C-order flat index
 int64_t dev_index(const gmes_sim* s, int c, int64_t i, int64_t j, int64_t k) {
@@ -145,7 +159,7 @@ void free_ade(gmes_sim* s) {
 // record table has at most 256 entries (other sims and kernels read the
 // 16-bit ids; for wider words the saving measured below the cost).
 int upload_byte_ids(gmes_sim* s) {
-  const bool narrow = s->g.active == 0x3Fu && s->g.dtype == GMES_F32 && s->recs.size() <= 256;
+  const bool narrow = is3d(s) && s->g.dtype == GMES_F32 && s->recs.size() <= 256;
   for (int c = 0; c < 6; ++c) {
     if (!narrow || !active(s, c)) {
       if (s->ids8[c]) { (void)dev_release(s->ids8[c]); s->ids8[c] = nullptr; }
@@ -373,6 +387,8 @@ bool main_args(const gmes_sim* s, bool electric, MainArgs<T>& m, int ext[3]) {
     for (int d = 0; d < 3; ++d) ext[d] = std::max<int>(ext[d], (int)s->shape[c][d]);
   }
   if (!any) return false;
+  m.kloop = !is3d(s) && ext[2] <= 2;
+  m.kext = ext[2];
   m.narrow = 1;
   for (int c = 0; c < 6; ++c) {
     m.ids[c] = s->ids[c];
@@ -398,7 +414,6 @@ bool main_args(const gmes_sim* s, bool electric, MainArgs<T>& m, int ext[3]) {
   return true;
 }
 
-bool is3d(const gmes_sim* s) { return s->g.active == 0x3Fu; }
 
 // Tile (bx along k, by along j) of the main kernels: the sim's own setting
 // (gmes_sim_set_tile, gmes_sim_autotune) or the default.  3-D: (j,k) tiles
@@ -407,10 +422,13 @@ bool is3d(const gmes_sim* s) { return s->g.active == 0x3Fu; }
 void tile_of(const gmes_sim* s, const int ext[3], int& bx, int& by) {
   bx = s->tile_bx;
   by = s->tile_by;
-  if (bx > 0 && by > 0 && (is3d(s) || bx <= next_pow2(ext[2]))) return;
+  if (bx > 0 && by > 0 && (is3d(s) || (bx <= next_pow2(ext[2]) && ext[2] > 2))) return;
   if (is3d(s)) {
     bx = std::min(128, next_pow2(s->g.nz));
     by = std::max(1, 256 / bx);
+  } else if (ext[2] <= 2) {
+    bx = 1;  // short rows: a thread per (i, j) walks k (MainArgs::kloop)
+    by = std::min(256, next_pow2(ext[1]));
   } else {
     bx = std::min(64, next_pow2(ext[2]));
     by = std::max(1, std::min(256 / bx, next_pow2(ext[1])));
@@ -432,10 +450,14 @@ void enqueue_main(gmes_sim* s, bool electric, hipStream_t st) {
   MainArgs<T> m;
   int ext[3];
   if (!main_args<T>(s, electric, m, ext)) return;
-  int bx, by;
-  tile_of(s, ext, bx, by);
   time_begin(s, electric ? 0 : 1, st);
-  launch_main_tile<T>(s, electric, m, ext, bx, by, st);
+  if (is_2z(s)) {
+    launch_main2z<T>(electric, m, ext, st);
+  } else {
+    int bx, by;
+    tile_of(s, ext, bx, by);
+    launch_main_tile<T>(s, electric, m, ext, bx, by, st);
+  }
   time_end(s, st);
 }
 
@@ -1589,7 +1611,7 @@ struct gmes_batch {
     int njobs = 0;
     int64_t nblocks = 0;
   };
-  Launch main3d[2], mainnd[2], src[2], ghost[2];  // [0] E, [1] H
+  Launch main3d[2], main2z[2], mainnd[2], src[2], ghost[2];  // [0] E, [1] H
   Launch ade[GMES_NCLS];                          // by dispersive class
   Launch dft;
   std::vector<int64_t> shape_sig;  // launch shapes the graph was captured with
@@ -1611,8 +1633,8 @@ void build_jobs(gmes_batch* b) {
   b->h_jobs.clear();
   for (int e = 0; e < 2; ++e) {
     const bool electric = e == 0;
-    std::vector<MainJob<T>> j3, jn;
-    int64_t nb3 = 0, nbn = 0;
+    std::vector<MainJob<T>> j3, j2, jn;
+    int64_t nb3 = 0, nb2 = 0, nbn = 0;
     std::vector<SrcJob<T>> js;
     int64_t nbs = 0;
     std::vector<GhostJob<T>> jg;
@@ -1631,8 +1653,15 @@ void build_jobs(gmes_batch* b) {
           J.block0 = nb3;
           nb3 += (int64_t)J.gx * J.gy * s->g.nx;
           j3.push_back(J);
+        } else if (is_2z(s)) {
+          J.bx = 256;
+          J.gx = (ext[1] + 255) / 256;
+          J.gy = (ext[0] + k2zRows - 1) / k2zRows;
+          J.block0 = nb2;
+          nb2 += (int64_t)J.gx * J.gy;
+          j2.push_back(J);
         } else {
-          J.gx = (ext[2] + bx - 1) / bx;
+          J.gx = J.m.kloop ? 1 : (ext[2] + bx - 1) / bx;
           J.gy = (ext[1] + tby - 1) / tby;
           J.block0 = nbn;
           nbn += (int64_t)J.gx * J.gy * ext[0];
@@ -1655,6 +1684,7 @@ void build_jobs(gmes_batch* b) {
       jg.push_back(G);
     }
     b->main3d[e] = {append_table(b->h_jobs, j3.data(), j3.size() * sizeof(MainJob<T>)), (int)j3.size(), nb3};
+    b->main2z[e] = {append_table(b->h_jobs, j2.data(), j2.size() * sizeof(MainJob<T>)), (int)j2.size(), nb2};
     b->mainnd[e] = {append_table(b->h_jobs, jn.data(), jn.size() * sizeof(MainJob<T>)), (int)jn.size(), nbn};
     b->src[e] = {append_table(b->h_jobs, js.data(), js.size() * sizeof(SrcJob<T>)), (int)js.size(), nbs};
     b->ghost[e] = {append_table(b->h_jobs, jg.data(), jg.size() * sizeof(GhostJob<T>)), (int)jg.size(), nbg};
@@ -1697,9 +1727,11 @@ void enqueue_batch_step_t(gmes_batch* b, hipStream_t st) {
       const auto& G = b->ghost[0];
       launch_ghost_batch<T>(job_ptr<GhostJob<T>>(b, G.off), G.njobs, G.nblocks, 0, st);
     }
-    launch_main_batch<T>(electric, true, job_ptr<MainJob<T>>(b, b->main3d[e].off), b->main3d[e].njobs,
+    launch_main_batch<T>(electric, kBatch3d, job_ptr<MainJob<T>>(b, b->main3d[e].off), b->main3d[e].njobs,
                          b->main3d[e].nblocks, st);
-    launch_main_batch<T>(electric, false, job_ptr<MainJob<T>>(b, b->mainnd[e].off), b->mainnd[e].njobs,
+    launch_main_batch<T>(electric, kBatch2z, job_ptr<MainJob<T>>(b, b->main2z[e].off), b->main2z[e].njobs,
+                         b->main2z[e].nblocks, st);
+    launch_main_batch<T>(electric, kBatchGeneric, job_ptr<MainJob<T>>(b, b->mainnd[e].off), b->mainnd[e].njobs,
                          b->mainnd[e].nblocks, st);
     if (electric)
       for (int cls = GMES_CLS_DRUDE; cls < GMES_NCLS; ++cls)
@@ -1826,7 +1858,7 @@ int gmes_batch_step(gmes_batch* b, int64_t nsteps) {
   // graph is still valid; table contents are read at run time
   std::vector<int64_t> sig;
   for (int e = 0; e < 2; ++e)
-    for (const auto* L : {&b->main3d[e], &b->mainnd[e], &b->src[e], &b->ghost[e]}) {
+    for (const auto* L : {&b->main3d[e], &b->main2z[e], &b->mainnd[e], &b->src[e], &b->ghost[e]}) {
       sig.push_back((int64_t)L->off);
       sig.push_back(L->njobs);
       sig.push_back(L->nblocks);

@@ -63,6 +63,9 @@ struct MainArgs {
   cplx<R> gph[3][2];  // ghost phase along a1, a2
   const uint8_t* ids8[6];  // the ids as bytes (real-FP32 3-D sims), when narrow
   int narrow;
+  // generic kernels: 1 when every row has at most 2 elements (kext); each
+  // thread then covers a whole row (bx = 1)
+  int kloop, kext;
 };
 
 template <class T>
@@ -166,8 +169,13 @@ struct AdeJob {
 
 template <class T>
 void launch_main(bool electric, const MainArgs<T>& m, int ext[3], int bx, int by, hipStream_t st);
+// collapsed-z 2-D kernels: i rows per thread (2 measured faster than 4,
+// and 1.4x the generic kernel at 4096^2 in FP32)
+constexpr int k2zRows = 2;
+// kinds of batched main launch
+constexpr int kBatchGeneric = 0, kBatch3d = 1, kBatch2z = 2;
 template <class T>
-void launch_main_batch(bool electric, bool three_d, const MainJob<T>* jobs, int njobs, int64_t nblocks,
+void launch_main_batch(bool electric, int kind, const MainJob<T>* jobs, int njobs, int64_t nblocks,
                        hipStream_t st);
 template <class T>
 void launch_src_batch(const SrcJob<T>* jobs, int njobs, int64_t nblocks, hipStream_t st);
@@ -177,6 +185,8 @@ template <class T>
 void launch_dft_batch(const DftJob<T>* jobs, int njobs, int64_t nblocks, hipStream_t st);
 template <class T>
 void launch_ghost_batch(const GhostJob<T>* jobs, int njobs, int64_t nblocks, int tick, hipStream_t st);
+template <class T>
+void launch_main2z(bool electric, const MainArgs<T>& m, int ext[3], hipStream_t st);
 template <class T>
 void launch_main3d(bool electric, const MainArgs<T>& m, int bx, int by, int ichunk, hipStream_t st);
 template <class T>

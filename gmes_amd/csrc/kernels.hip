@@ -357,9 +357,137 @@ __device__ __forceinline__ void main_generic_cell(const MainArgs<T>& m, int i, i
 #undef GMES_MAIN_COMP
 }
 
+// kloop: rows of at most 2 elements (a collapsed z axis, as in TMz/TEz):
+// threads run along j and each walks the few k of its row, so no lane sits
+// on a row position that does not exist or holds only ghosts.
 template <bool E, class T>
 __global__ void __launch_bounds__(256) k_main(MainArgs<T> m) {
-  main_generic_cell<E, T>(m, blockIdx.z, blockIdx.y * blockDim.y + threadIdx.y, blockIdx.x * blockDim.x + threadIdx.x);
+  const int i = blockIdx.z, j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (m.kloop) {
+    for (int k = 0; k < m.kext; ++k) main_generic_cell<E, T>(m, i, j, k);
+  } else {
+    main_generic_cell<E, T>(m, i, j, blockIdx.x * blockDim.x + threadIdx.x);
+  }
+}
+
+// Record table staging: the table is copied as 16-byte chunks, one per
+// thread, loaded into registers first and written to LDS later.
+template <class R>
+struct RecStage {
+  static constexpr int kChunks = sizeof(DevRec<R>) / 16;
+  static_assert(sizeof(DevRec<R>) % 16 == 0, "record size must be a multiple of 16 bytes");
+  i4_t v;
+  int t;
+  bool use_lds;
+  __device__ __forceinline__ void load(const DevRec<R>* g, int nrec, int tid, int nthreads) {
+    t = tid;
+    use_lds = nrec * kChunks <= nthreads && nrec <= kRecLds;
+    if (use_lds && t < nrec * kChunks) v = gld(reinterpret_cast<const i4_t*>(g) + t);
+  }
+  __device__ __forceinline__ void commit(DevRec<R>* lds, int nrec) {
+    if (!use_lds) return;
+    if (t < nrec * kChunks) reinterpret_cast<i4_t*>(lds)[t] = v;
+    __syncthreads();
+  }
+};
+
+// 2-D modes with a collapsed z axis (TMz, TEz): every row holds one or two
+// elements, so the generic kernel's threads (one per element) would each
+// run one short dependent chain (id, record, curl operands, update).  Here
+// a thread owns (i, j) for N consecutive i and every component of the
+// field type at its one updated z plane (k = 0 for E and for rows of one
+// element, k = 1 for the H components whose k = 0 plane is the ghost).  All
+// ids, old values and curl operands of its cells are loaded first; ghost
+// cells, whose neighbours lie outside the arrays, load no operands.  The
+// record table is staged into LDS meanwhile, and only then do the updates
+// run.
+template <class T, int N>
+struct Cells2z {
+  T f[3][N], d1[3][N], d2[3][N];
+  int flat[3][N];
+  uint16_t id[3][N];
+  bool upd[3][N];  // an in-range, non-ghost cell of an active component
+};
+
+// Ghost cells are the last plane (E) or the first plane (H) along each
+// transverse axis whose extent exceeds 1.
+template <int C, class T>
+__device__ __forceinline__ bool ghost_cell(const FieldArgs<T>& a, int i, int j, int k) {
+  constexpr bool E = C < 3;
+  const int ix[3] = {i, j, k};
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    if (ax == C % 3 || a.s[C][ax] == 1) continue;
+    if (E ? ix[ax] == a.s[C][ax] - 1 : ix[ax] == 0) return true;
+  }
+  return false;
+}
+
+template <int C, class T, int N>
+__device__ __forceinline__ void load2z(const MainArgs<T>& m, Cells2z<T, N>& q, int i0, int j) {
+  constexpr int c = C % 3;
+  const FieldArgs<T>& a = m.fa;
+  const bool on = ((a.active >> C) & 1u) && j < a.s[C][1];
+  const int kc = (a.s[C][2] == 1 || C < 3) ? 0 : 1;
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    const int i = i0 + n;
+    q.upd[c][n] = false;
+    if (!on || i >= a.s[C][0]) continue;
+    const int flat = didx(a, C, i, j, kc);
+    q.flat[c][n] = flat;
+    q.id[c][n] = gld(m.ids[C] + flat);
+    q.f[c][n] = gld(a.f[C] + flat);
+    if (ghost_cell<C>(a, i, j, kc)) continue;
+    q.upd[c][n] = true;
+    curl_terms<C>(a, i, j, kc, q.d1[c][n], q.d2[c][n]);
+  }
+}
+
+template <int C, class T, int N>
+__device__ __forceinline__ void update2z(const MainArgs<T>& m, const DevRec<typename real_of<T>::type>* recs,
+                                         const Cells2z<T, N>& q, int i0, int j) {
+  constexpr int c = C % 3;
+  const FieldArgs<T>& a = m.fa;
+  const int kc = (a.s[C][2] == 1 || C < 3) ? 0 : 1;
+  T* st = static_cast<T*>(m.pml[C]);
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    if (!q.upd[c][n]) continue;
+    const int flat = q.flat[c][n];
+    (void)store_cell<C>(m, recs[q.id[c][n]], a.f[C], st, flat, i0 + n, j, kc, q.f[c][n], q.d1[c][n],
+                        q.d2[c][n]);
+  }
+}
+
+template <bool E, class T, int N>
+__device__ __forceinline__ void main2z_tile(const MainArgs<T>& m, DevRec<typename real_of<T>::type>* srec, int i0,
+                                            int j, int tid, int nthreads) {
+  using R = typename real_of<T>::type;
+  RecStage<R> stage;
+  stage.load(m.recs, m.nrec, tid, nthreads);
+  constexpr int C0 = E ? 0 : 3;
+  Cells2z<T, N> q;
+  load2z<C0 + 0, T, N>(m, q, i0, j);
+  load2z<C0 + 1, T, N>(m, q, i0, j);
+  load2z<C0 + 2, T, N>(m, q, i0, j);
+  stage.commit(srec, m.nrec);
+  if (stage.use_lds) {
+    update2z<C0 + 0, T, N>(m, srec, q, i0, j);
+    update2z<C0 + 1, T, N>(m, srec, q, i0, j);
+    update2z<C0 + 2, T, N>(m, srec, q, i0, j);
+  } else {
+    update2z<C0 + 0, T, N>(m, m.recs, q, i0, j);
+    update2z<C0 + 1, T, N>(m, m.recs, q, i0, j);
+    update2z<C0 + 2, T, N>(m, m.recs, q, i0, j);
+  }
+}
+
+template <bool E, class T>
+__global__ void __launch_bounds__(256) k_main2z(MainArgs<T> m) {
+  __shared__ DevRec<typename real_of<T>::type> srec[kRecLds];
+  main2z_tile<E, T, k2zRows>(m, srec, blockIdx.y * k2zRows, blockIdx.x * blockDim.x + threadIdx.x, threadIdx.x,
+                             blockDim.x);
 }
 
 // 3-D main kernel (all six components active).  Thread (i,j,k) owns the
@@ -504,26 +632,6 @@ __device__ __forceinline__ void update_h(const MainArgs<T>& m, const DevRec<type
   }
 }
 
-// Record table staging: the table is copied as 16-byte chunks, one per
-// thread, loaded into registers first and written to LDS later.
-template <class R>
-struct RecStage {
-  static constexpr int kChunks = sizeof(DevRec<R>) / 16;
-  static_assert(sizeof(DevRec<R>) % 16 == 0, "record size must be a multiple of 16 bytes");
-  i4_t v;
-  int t;
-  bool use_lds;
-  __device__ __forceinline__ void load(const DevRec<R>* g, int nrec, int tid, int nthreads) {
-    t = tid;
-    use_lds = nrec * kChunks <= nthreads && nrec <= kRecLds;
-    if (use_lds && t < nrec * kChunks) v = gld(reinterpret_cast<const i4_t*>(g) + t);
-  }
-  __device__ __forceinline__ void commit(DevRec<R>* lds, int nrec) {
-    if (!use_lds) return;
-    if (t < nrec * kChunks) reinterpret_cast<i4_t*>(lds)[t] = v;
-    __syncthreads();
-  }
-};
 
 // One tile of the 3-D update: thread (k, j) of planes i0 .. i0+N-1; tid and
 // nthreads are the thread's rank in the block and the block size (the
@@ -638,7 +746,23 @@ __global__ void __launch_bounds__(256) k_main_batch(const MainJob<T>* jobs, int 
   const int bxi = local % J.gx, byi = (local / J.gx) % J.gy, bzi = local / (J.gx * J.gy);
   const int tid = threadIdx.x;
   const int tx = tid & (J.bx - 1), ty = tid / J.bx;
-  main_generic_cell<E, T>(J.m, bzi, byi * (256 / J.bx) + ty, bxi * J.bx + tx);
+  const int j = byi * (256 / J.bx) + ty;
+  if (J.m.kloop) {
+    for (int k = 0; k < J.m.kext; ++k) main_generic_cell<E, T>(J.m, bzi, j, k);
+  } else {
+    main_generic_cell<E, T>(J.m, bzi, j, bxi * J.bx + tx);
+  }
+}
+
+// Collapsed-z 2-D members: a job's blocks are gx blocks of 256 j by gy
+// groups of k2zRows i rows.
+template <bool E, class T>
+__global__ void __launch_bounds__(256) k_main2z_batch(const MainJob<T>* jobs, int njobs) {
+  __shared__ DevRec<typename real_of<T>::type> srec[kRecLds];
+  const MainJob<T>& J = jobs[find_job(jobs, njobs, blockIdx.x)];
+  const int local = (int)(blockIdx.x - J.block0);
+  const int bxi = local % J.gx, byi = local / J.gx;
+  main2z_tile<E, T, k2zRows>(J.m, srec, byi * k2zRows, bxi * 256 + (int)threadIdx.x, threadIdx.x, 256);
 }
 
 // Batched launches: block -> job by binary search on the jobs' first
@@ -1079,9 +1203,17 @@ __global__ void __launch_bounds__(256) k_ghost_batch(const GhostJob<T>* jobs, in
 
 // --------------------------------------------------------------- launchers
 template <class T>
+void launch_main2z(bool electric, const MainArgs<T>& m, int ext[3], hipStream_t st) {
+  const int bj = ext[1] >= 256 ? 256 : (ext[1] >= 128 ? 128 : 64);
+  dim3 block(bj), grid((ext[1] + bj - 1) / bj, (ext[0] + k2zRows - 1) / k2zRows);
+  if (electric) hipLaunchKernelGGL((k_main2z<true, T>), grid, block, 0, st, m);
+  else hipLaunchKernelGGL((k_main2z<false, T>), grid, block, 0, st, m);
+}
+
+template <class T>
 void launch_main(bool electric, const MainArgs<T>& m, int ext[3], int bx, int by, hipStream_t st) {
   dim3 block(bx, by, 1);
-  dim3 grid((ext[2] + bx - 1) / bx, (ext[1] + by - 1) / by, ext[0]);
+  dim3 grid(m.kloop ? 1 : (ext[2] + bx - 1) / bx, (ext[1] + by - 1) / by, ext[0]);
   if (electric) hipLaunchKernelGGL((k_main<true, T>), grid, block, 0, st, m);
   else hipLaunchKernelGGL((k_main<false, T>), grid, block, 0, st, m);
 }
@@ -1118,13 +1250,16 @@ void launch_ade(int cls, int c, const AdeArgs<T>& m, hipStream_t st) {
 }
 
 template <class T>
-void launch_main_batch(bool electric, bool three_d, const MainJob<T>* jobs, int njobs, int64_t nblocks,
+void launch_main_batch(bool electric, int kind, const MainJob<T>* jobs, int njobs, int64_t nblocks,
                        hipStream_t st) {
   if (njobs == 0 || nblocks == 0) return;
   const dim3 grid((unsigned)nblocks), block(256);
-  if (three_d) {
+  if (kind == kBatch3d) {
     if (electric) hipLaunchKernelGGL((k_main3d_batch<true, T>), grid, block, 0, st, jobs, njobs);
     else hipLaunchKernelGGL((k_main3d_batch<false, T>), grid, block, 0, st, jobs, njobs);
+  } else if (kind == kBatch2z) {
+    if (electric) hipLaunchKernelGGL((k_main2z_batch<true, T>), grid, block, 0, st, jobs, njobs);
+    else hipLaunchKernelGGL((k_main2z_batch<false, T>), grid, block, 0, st, jobs, njobs);
   } else {
     if (electric) hipLaunchKernelGGL((k_main_batch<true, T>), grid, block, 0, st, jobs, njobs);
     else hipLaunchKernelGGL((k_main_batch<false, T>), grid, block, 0, st, jobs, njobs);
@@ -1178,11 +1313,12 @@ void launch_ghost(const GhostArgs<T>& m, int64_t total, hipStream_t st) {
 #define GMES_INSTANTIATE(T)                                                                     \
   template void launch_main<T>(bool, const MainArgs<T>&, int[3], int, int, hipStream_t);       \
   template void launch_main3d<T>(bool, const MainArgs<T>&, int, int, int, hipStream_t);        \
+  template void launch_main2z<T>(bool, const MainArgs<T>&, int[3], hipStream_t);               \
   template void launch_ade<T>(int, int, const AdeArgs<T>&, hipStream_t);                       \
   template void launch_src<T>(const SrcArgs<T>&, hipStream_t);                                  \
   template void launch_probe<T>(const ProbeArgs<T>&, hipStream_t);                              \
   template void launch_ghost<T>(const GhostArgs<T>&, int64_t, hipStream_t);                     \
-  template void launch_main_batch<T>(bool, bool, const MainJob<T>*, int, int64_t, hipStream_t);  \
+  template void launch_main_batch<T>(bool, int, const MainJob<T>*, int, int64_t, hipStream_t);   \
   template void launch_src_batch<T>(const SrcJob<T>*, int, int64_t, hipStream_t);               \
   template void launch_ade_batch<T>(int, const AdeJob<T>*, int, int64_t, hipStream_t);          \
   template void launch_dft_batch<T>(const DftJob<T>*, int, int64_t, hipStream_t);               \
