@@ -169,9 +169,9 @@ struct AdeJob {
 
 template <class T>
 void launch_main(bool electric, const MainArgs<T>& m, int ext[3], int bx, int by, hipStream_t st);
-// collapsed-z 2-D kernels: i rows per thread (2 measured faster than 4,
-// and 1.4x the generic kernel at 4096^2 in FP32)
-constexpr int k2zRows = 2;
+// collapsed-z 2-D kernels: i rows per thread (1 measured fastest of 1, 2
+// and 4 at 4096^2; 1.4x the generic kernel in FP32)
+constexpr int k2zRows = 1;
 // kinds of batched main launch
 constexpr int kBatchGeneric = 0, kBatch3d = 1, kBatch2z = 2;
 template <class T>
