@@ -389,6 +389,10 @@ bool main_args(const gmes_sim* s, bool electric, MainArgs<T>& m, int ext[3]) {
   if (!any) return false;
   m.kloop = !is3d(s) && ext[2] <= 2;
   m.kext = ext[2];
+  m.cpml_mask = 0;
+  for (int c = 0; c < 6; ++c)
+    if (s->pml_state[c] && s->has_cls[c][GMES_CLS_CPML]) m.cpml_mask |= 1u << c;
+  for (int a = 0; a < 3; ++a) m.prun[a] = s->prun[a];
   m.narrow = 1;
   for (int c = 0; c < 6; ++c) {
     m.ids[c] = s->ids[c];
@@ -1190,6 +1194,19 @@ int gmes_sim_commit(gmes_sim* s) {
   int rc = s->fp64 ? upload_tables<double>(s) : upload_tables<float>(s);
   if (rc) return rc;
   if ((rc = upload_byte_ids(s))) return rc;
+  // the coordinate runs where no CPML record's psi can be nonzero
+  for (int a = 0; a < 3; ++a) {
+    int lo = 0, hi = 0x7FFF;
+    bool any = false;
+    for (const gmes_mat_rec& r : s->recs) {
+      if (r.cls != GMES_CLS_CPML) continue;
+      const int32_t run = (r.prof >= 0 && r.prof < s->nprof) ? zero_c_run(s, r.prof, a) : 0;
+      any = true;
+      lo = std::max(lo, (int)(run & 0xFFFF));
+      hi = std::min(hi, (int)(run >> 16));
+    }
+    s->prun[a] = (any && lo < hi) ? (lo | (hi << 16)) : 0;
+  }
   s->committed = true;
   destroy_graph(s);
   decide_fusion(s);

@@ -66,6 +66,10 @@ struct MainArgs {
   // generic kernels: 1 when every row has at most 2 elements (kext); each
   // thread then covers a whole row (bx = 1)
   int kloop, kext;
+  // real-FP32 3-D kernels: components with CPML state (bit c), and per axis
+  // the coordinate run (lo | hi << 16) where every CPML profile's c is zero
+  unsigned cpml_mask;
+  int prun[3];
 };
 
 template <class T>

@@ -160,6 +160,10 @@ __device__ __forceinline__ cplx<float> fma_(float a, cplx<float> b, cplx<float> 
   return {__builtin_fmaf(a, b.re, c.re), __builtin_fmaf(a, b.im, c.im)};
 }
 
+// PRE: the CPML psi words pre1, pre2 of a real-FP32 cell were loaded with
+// the cell's other operands (prefetch_psi); otherwise update_value loads
+// them.
+//
 // CPML psi words that can be nonzero.  Along an axis where the profile's
 // c coefficient is exactly zero (outside that axis's layers: sigma = 0),
 // psi starts at +0 and stays +0, since b*(+0) + (+-0)*x = +0; computing it
@@ -181,9 +185,10 @@ __device__ __forceinline__ bool psi_live(const DevRec<R>& rec, const int ix[3]) 
 //   src/pw_cpml.hh:127-132,339-344, src/pw_upml.hh:124-128,328-332.
 //   FP32 uses reciprocals the host computed in FP64 (1/d, 1/kappa in the
 //   CPML profile, 1/eps in the record) and explicit FMAs.
-template <int C, class T>
+template <int C, class T, bool PRE>
 __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typename real_of<T>::type>& rec,
-                                          T* __restrict__ st, int flat, int i, int j, int k, T f, T D1, T D2) {
+                                          T* __restrict__ st, int flat, int i, int j, int k, T f, T D1, T D2,
+                                          T pre1, T pre2) {
   using R = typename real_of<T>::type;
   constexpr bool E = C < 3;
   const int cls = rec.cls;
@@ -210,8 +215,8 @@ __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typ
         // traffic (measured).  A dead psi is read from element 0 of its
         // array instead, a guard word that is always +0 (one cache line
         // for all such lanes), and its +0 is stored back unconditionally.
-        const T o1 = gld(st + (u1 ? flat : 0));
-        const T o2 = gld(st + m.dsize[C] + (u2 ? flat : 0));
+        const T o1 = PRE ? pre1 : gld(st + (u1 ? flat : 0));
+        const T o2 = PRE ? pre2 : gld(st + m.dsize[C] + (u2 ? flat : 0));
         psi1 = fma_(p1[1], a, vscl(p1[0], o1));
         psi2 = fma_(p2[1], b, vscl(p2[0], o2));
         gst(st + flat, psi1);
@@ -268,10 +273,10 @@ __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typ
 // Update of one cell of component C: Dummy (no-op), Const (src/pw_const.hh
 // :96,198), Dielectric, CPML, UPML.  ADE classes are left to their kernels.
 // f is the cell's old value, loaded by the caller.
-template <int C, class T>
+template <int C, class T, bool PRE = false>
 __device__ __forceinline__ T store_cell(const MainArgs<T>& m, const DevRec<typename real_of<T>::type>& rec,
                                         T* __restrict__ F, T* __restrict__ st, int flat, int i, int j, int k,
-                                        T f, T D1, T D2) {
+                                        T f, T D1, T D2, T pre1 = T{}, T pre2 = T{}) {
   const int cls = rec.cls;
   if (cls == GMES_CLS_DUMMY || cls >= GMES_CLS_DRUDE) return f;
   if (cls == GMES_CLS_CONST) {
@@ -279,7 +284,7 @@ __device__ __forceinline__ T store_cell(const MainArgs<T>& m, const DevRec<typen
     st_nt(F + flat, v);
     return v;
   }
-  const T v = update_value<C>(m, rec, st, flat, i, j, k, f, D1, D2);
+  const T v = update_value<C, T, PRE>(m, rec, st, flat, i, j, k, f, D1, D2, pre1, pre2);
   st_nt(F + flat, v);
   return v;
 }
@@ -500,15 +505,34 @@ __global__ void __launch_bounds__(256) k_main2z(MainArgs<T> m) {
 // issued first, through non-aliasing pointers; the record table is staged
 // into LDS while those loads are in flight, and only then do the dependent
 // record lookups and updates run.
+// Real FP32, cells of a component with CPML state: its two psi words are
+// loaded with the cell's other operands, so a CPML cell needs no further
+// dependent load after its record.  Outside the coordinate runs where
+// every CPML profile's c is zero (m.prun) the word is live; inside, it is
+// +0 and is read from the array's always-zero guard word instead.
+template <int C, class T>
+__device__ __forceinline__ void prefetch_psi(const MainArgs<T>& m, int flat, const int ix[3], T& o1, T& o2) {
+  o1 = o2 = T{};
+  if (!((m.cpml_mask >> C) & 1u)) return;
+  const T* st = static_cast<const T*>(m.pml[C]);
+  const int r1 = m.prun[Ax<C>::a1], r2 = m.prun[Ax<C>::a2];
+  const int x1 = ix[Ax<C>::a1], x2 = ix[Ax<C>::a2];
+  const bool u1 = x1 < (r1 & 0xFFFF) || x1 >= (r1 >> 16);
+  const bool u2 = x2 < (r2 & 0xFFFF) || x2 >= (r2 >> 16);
+  o1 = gld(st + (u1 ? flat : 0));
+  o2 = gld(st + m.dsize[C] + (u2 ? flat : 0));
+}
+
 template <class T, int N>
 struct Planes {
   T c[7][N];     // curl operands (see load_e / load_h)
   T o[3][N];     // old values of the three updated components
   uint16_t r[3][N];
+  T s[3][2][N];  // prefetched CPML psi words (real FP32)
 };
 
 template <class T, int N>
-__device__ __forceinline__ void load_e(Planes<T, N>& q, int i0, int j, int k, const int* B, const int* P,
+__device__ __forceinline__ void load_e(const MainArgs<T>& m, Planes<T, N>& q, int i0, int j, int k, const int* B, const int* P,
                                        const int* W, const T* __restrict__ hx, const T* __restrict__ hy,
                                        const T* __restrict__ hz, const T* __restrict__ ex, const T* __restrict__ ey,
                                        const T* __restrict__ ez, const uint16_t* __restrict__ idx,
@@ -539,6 +563,12 @@ __device__ __forceinline__ void load_e(Planes<T, N>& q, int i0, int j, int k, co
       q.r[1][n] = ld_nt(idy + fy);
       q.r[2][n] = ld_nt(idz + fz);
     }
+    if constexpr (sizeof(T) == 4) {
+      const int ix[3] = {i, j, k};
+      prefetch_psi<0>(m, fx, ix, q.s[0][0][n], q.s[0][1][n]);
+      prefetch_psi<1>(m, fy, ix, q.s[1][0][n], q.s[1][1][n]);
+      prefetch_psi<2>(m, fz, ix, q.s[2][0][n], q.s[2][1][n]);
+    }
   }
 }
 
@@ -550,17 +580,21 @@ __device__ __forceinline__ void update_e(const MainArgs<T>& m, const DevRec<type
                                          const int* W, T* __restrict__ ex, T* __restrict__ ey, T* __restrict__ ez,
                                          T* __restrict__ sx, T* __restrict__ sy, T* __restrict__ sz, T& hz_prev,
                                          T& hy_prev) {
+  constexpr bool kPre = sizeof(T) == 4;  // psi words prefetched (load_e / load_h)
 #pragma unroll
   for (int n = 0; n < N; ++n) {
     const int i = i0 + n;
     const int fx = B[0] + i * P[0] + j * W[0] + k, fy = B[1] + i * P[1] + j * W[1] + k;
     const int fz = B[2] + i * P[2] + j * W[2] + k;
-    const T vx = store_cell<0>(m, recs[q.r[0][n]], ex, sx, fx, i, j, k, q.o[0][n], vsub(q.c[5][n], q.c[6][n]),
-                               vsub(q.c[3][n], q.c[4][n]));
-    const T vy = store_cell<1>(m, recs[q.r[1][n]], ey, sy, fy, i, j, k, q.o[1][n], vsub(q.c[0][n], q.c[2][n]),
-                               vsub(q.c[5][n], hz_prev));
-    const T vz = store_cell<2>(m, recs[q.r[2][n]], ez, sz, fz, i, j, k, q.o[2][n], vsub(q.c[3][n], hy_prev),
-                               vsub(q.c[0][n], q.c[1][n]));
+    const T vx = store_cell<0, T, kPre>(m, recs[q.r[0][n]], ex, sx, fx, i, j, k, q.o[0][n],
+                                        vsub(q.c[5][n], q.c[6][n]), vsub(q.c[3][n], q.c[4][n]),
+                                        q.s[0][0][n], q.s[0][1][n]);
+    const T vy = store_cell<1, T, kPre>(m, recs[q.r[1][n]], ey, sy, fy, i, j, k, q.o[1][n],
+                                        vsub(q.c[0][n], q.c[2][n]), vsub(q.c[5][n], hz_prev),
+                                        q.s[1][0][n], q.s[1][1][n]);
+    const T vz = store_cell<2, T, kPre>(m, recs[q.r[2][n]], ez, sz, fz, i, j, k, q.o[2][n],
+                                        vsub(q.c[3][n], hy_prev), vsub(q.c[0][n], q.c[1][n]),
+                                        q.s[2][0][n], q.s[2][1][n]);
     if (m.fuse && (i == 0 || j == 0 || k == 0)) {
       const int ix[3] = {i, j, k};
       fused_ghost<T>(m, 0, ex, ix, vx);
@@ -573,7 +607,7 @@ __device__ __forceinline__ void update_e(const MainArgs<T>& m, const DevRec<type
 }
 
 template <class T, int N>
-__device__ __forceinline__ void load_h(Planes<T, N>& q, int i0, int j, int k, const int* B, const int* P,
+__device__ __forceinline__ void load_h(const MainArgs<T>& m, Planes<T, N>& q, int i0, int j, int k, const int* B, const int* P,
                                        const int* W, const T* __restrict__ ex, const T* __restrict__ ey,
                                        const T* __restrict__ ez, const T* __restrict__ hx, const T* __restrict__ hy,
                                        const T* __restrict__ hz, const uint16_t* __restrict__ idx,
@@ -605,6 +639,12 @@ __device__ __forceinline__ void load_h(Planes<T, N>& q, int i0, int j, int k, co
       q.r[1][n] = ld_nt(idy + fy);
       q.r[2][n] = ld_nt(idz + fz);
     }
+    if constexpr (sizeof(T) == 4) {
+      const int ixx[3] = {i, j + 1, k + 1}, ixy[3] = {i + 1, j, k + 1}, ixz[3] = {i + 1, j + 1, k};
+      prefetch_psi<3>(m, fx, ixx, q.s[0][0][n], q.s[0][1][n]);
+      prefetch_psi<4>(m, fy, ixy, q.s[1][0][n], q.s[1][1][n]);
+      prefetch_psi<5>(m, fz, ixz, q.s[2][0][n], q.s[2][1][n]);
+    }
   }
 }
 
@@ -615,18 +655,21 @@ __device__ __forceinline__ void update_h(const MainArgs<T>& m, const DevRec<type
                                          const int* W, T* __restrict__ hx, T* __restrict__ hy, T* __restrict__ hz,
                                          T* __restrict__ sx, T* __restrict__ sy, T* __restrict__ sz, T& ey_cur,
                                          T& ez_cur) {
+  constexpr bool kPre = sizeof(T) == 4;  // psi words prefetched (load_e / load_h)
 #pragma unroll
   for (int n = 0; n < N; ++n) {
     const int i = i0 + n;
     const int fx = B[3] + i * P[3] + (j + 1) * W[3] + k + 1;
     const int fy = B[4] + (i + 1) * P[4] + j * W[4] + k + 1;
     const int fz = B[5] + (i + 1) * P[5] + (j + 1) * W[5] + k;
-    const T vx = store_cell<3>(m, recs[q.r[0][n]], hx, sx, fx, i, j + 1, k + 1, q.o[0][n], vsub(q.c[5][n], ez_cur),
-                               vsub(q.c[3][n], ey_cur));
-    const T vy = store_cell<4>(m, recs[q.r[1][n]], hy, sy, fy, i + 1, j, k + 1, q.o[1][n],
-                               vsub(q.c[1][n], q.c[0][n]), vsub(q.c[6][n], ez_cur));
-    const T vz = store_cell<5>(m, recs[q.r[2][n]], hz, sz, fz, i + 1, j + 1, k, q.o[2][n], vsub(q.c[4][n], ey_cur),
-                               vsub(q.c[2][n], q.c[0][n]));
+    const T vx = store_cell<3, T, kPre>(m, recs[q.r[0][n]], hx, sx, fx, i, j + 1, k + 1, q.o[0][n],
+                                        vsub(q.c[5][n], ez_cur), vsub(q.c[3][n], ey_cur), q.s[0][0][n], q.s[0][1][n]);
+    const T vy = store_cell<4, T, kPre>(m, recs[q.r[1][n]], hy, sy, fy, i + 1, j, k + 1, q.o[1][n],
+                                        vsub(q.c[1][n], q.c[0][n]), vsub(q.c[6][n], ez_cur),
+                                        q.s[1][0][n], q.s[1][1][n]);
+    const T vz = store_cell<5, T, kPre>(m, recs[q.r[2][n]], hz, sz, fz, i + 1, j + 1, k, q.o[2][n],
+                                        vsub(q.c[4][n], ey_cur), vsub(q.c[2][n], q.c[0][n]),
+                                        q.s[2][0][n], q.s[2][1][n]);
     ey_cur = q.c[4][n];
     ez_cur = q.c[6][n];
   }
@@ -667,11 +710,11 @@ __device__ __forceinline__ void main3d_tile(const MainArgs<T>& m, DevRec<typenam
     T u0 = T{}, u1 = T{};
     if (act) {
       if constexpr (E) {
-        load_e<T, N>(q, i0, j, k, B, P, W, f[3], f[4], f[5], f[0], f[1], f[2], ida, idb, idc, i8a, i8b, i8c, m.narrow != 0);
+        load_e<T, N>(m, q, i0, j, k, B, P, W, f[3], f[4], f[5], f[0], f[1], f[2], ida, idb, idc, i8a, i8b, i8c, m.narrow != 0);
         u0 = gld(f[5] + B[5] + i0 * P[5] + (j + 1) * W[5] + k);
         u1 = gld(f[4] + B[4] + i0 * P[4] + j * W[4] + k + 1);
       } else {
-        load_h<T, N>(q, i0, j, k, B, P, W, f[0], f[1], f[2], f[3], f[4], f[5], ida, idb, idc, i8a, i8b, i8c, m.narrow != 0);
+        load_h<T, N>(m, q, i0, j, k, B, P, W, f[0], f[1], f[2], f[3], f[4], f[5], ida, idb, idc, i8a, i8b, i8c, m.narrow != 0);
         u0 = gld(f[1] + B[1] + i0 * P[1] + j * W[1] + k);
         u1 = gld(f[2] + B[2] + i0 * P[2] + j * W[2] + k);
       }
@@ -704,10 +747,10 @@ __device__ __forceinline__ void main3d_tile(const MainArgs<T>& m, DevRec<typenam
     for (int i = i0; i < nx; ++i) {
       Planes<T, 1> q;
       if constexpr (E) {
-        load_e<T, 1>(q, i, j, k, B, P, W, f[3], f[4], f[5], f[0], f[1], f[2], ida, idb, idc, i8a, i8b, i8c, m.narrow != 0);
+        load_e<T, 1>(m, q, i, j, k, B, P, W, f[3], f[4], f[5], f[0], f[1], f[2], ida, idb, idc, i8a, i8b, i8c, m.narrow != 0);
         update_e<T, 1>(m, recs, q, i, j, k, B, P, W, f[0], f[1], f[2], sx, sy, sz, u0, u1);
       } else {
-        load_h<T, 1>(q, i, j, k, B, P, W, f[0], f[1], f[2], f[3], f[4], f[5], ida, idb, idc, i8a, i8b, i8c, m.narrow != 0);
+        load_h<T, 1>(m, q, i, j, k, B, P, W, f[0], f[1], f[2], f[3], f[4], f[5], ida, idb, idc, i8a, i8b, i8c, m.narrow != 0);
         update_h<T, 1>(m, recs, q, i, j, k, B, P, W, f[3], f[4], f[5], sx, sy, sz, u0, u1);
       }
     }
