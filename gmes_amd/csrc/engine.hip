@@ -1628,9 +1628,15 @@ struct gmes_batch {
     int njobs = 0;
     int64_t nblocks = 0;
   };
-  Launch main3d[2], main2z[2], mainnd[2], src[2], ghost[2];  // [0] E, [1] H
-  Launch ade[GMES_NCLS];                          // by dispersive class
-  Launch dft;
+  // the launches of one step of a group of sims
+  struct JobSet {
+    Launch main3d[2], main2z[2], mainnd[2], src[2], ghost[2];  // [0] E, [1] H
+    Launch ade[GMES_NCLS];                                      // by dispersive class
+    Launch dft;
+  };
+  JobSet set;                      // the members
+  JobSet aux_set;                  // their auxiliary (TFSF) grids, stepped inside the members' step
+  std::vector<gmes_sim*> aux_sims;
   std::vector<int64_t> shape_sig;  // launch shapes the graph was captured with
 };
 
@@ -1646,8 +1652,7 @@ size_t append_table(std::vector<char>& h, const void* p, size_t bytes) {
 // Job tables of every batched launch.  The tiles follow each member's own
 // setting (gmes_sim_set_tile / gmes_sim_autotune), with by = 256 / bx.
 template <class T>
-void build_jobs(gmes_batch* b) {
-  b->h_jobs.clear();
+void build_jobs(gmes_batch* b, const std::vector<gmes_sim*>& sims, gmes_batch::JobSet& out) {
   for (int e = 0; e < 2; ++e) {
     const bool electric = e == 0;
     std::vector<MainJob<T>> j3, j2, jn;
@@ -1656,7 +1661,7 @@ void build_jobs(gmes_batch* b) {
     int64_t nbs = 0;
     std::vector<GhostJob<T>> jg;
     int64_t nbg = 0;
-    for (gmes_sim* s : b->sims) {
+    for (gmes_sim* s : sims) {
       MainJob<T> J{};
       int ext[3];
       if (main_args<T>(s, electric, J.m, ext)) {
@@ -1700,16 +1705,16 @@ void build_jobs(gmes_batch* b) {
       nbg += std::max<int64_t>(1, (total + 255) / 256);  // >= 1: the block that ticks the clock
       jg.push_back(G);
     }
-    b->main3d[e] = {append_table(b->h_jobs, j3.data(), j3.size() * sizeof(MainJob<T>)), (int)j3.size(), nb3};
-    b->main2z[e] = {append_table(b->h_jobs, j2.data(), j2.size() * sizeof(MainJob<T>)), (int)j2.size(), nb2};
-    b->mainnd[e] = {append_table(b->h_jobs, jn.data(), jn.size() * sizeof(MainJob<T>)), (int)jn.size(), nbn};
-    b->src[e] = {append_table(b->h_jobs, js.data(), js.size() * sizeof(SrcJob<T>)), (int)js.size(), nbs};
-    b->ghost[e] = {append_table(b->h_jobs, jg.data(), jg.size() * sizeof(GhostJob<T>)), (int)jg.size(), nbg};
+    out.main3d[e] = {append_table(b->h_jobs, j3.data(), j3.size() * sizeof(MainJob<T>)), (int)j3.size(), nb3};
+    out.main2z[e] = {append_table(b->h_jobs, j2.data(), j2.size() * sizeof(MainJob<T>)), (int)j2.size(), nb2};
+    out.mainnd[e] = {append_table(b->h_jobs, jn.data(), jn.size() * sizeof(MainJob<T>)), (int)jn.size(), nbn};
+    out.src[e] = {append_table(b->h_jobs, js.data(), js.size() * sizeof(SrcJob<T>)), (int)js.size(), nbs};
+    out.ghost[e] = {append_table(b->h_jobs, jg.data(), jg.size() * sizeof(GhostJob<T>)), (int)jg.size(), nbg};
   }
   for (int cls = GMES_CLS_DRUDE; cls < GMES_NCLS; ++cls) {
     std::vector<AdeJob<T>> ja;
     int64_t nba = 0;
-    for (gmes_sim* s : b->sims)
+    for (gmes_sim* s : sims)
       for (int c = 0; c < 3; ++c) {
         if (s->ade[c][cls].n == 0) continue;
         AdeJob<T> A{};
@@ -1719,12 +1724,12 @@ void build_jobs(gmes_batch* b) {
         nba += (A.a.n + 255) / 256;
         ja.push_back(A);
       }
-    b->ade[cls] = {append_table(b->h_jobs, ja.data(), ja.size() * sizeof(AdeJob<T>)), (int)ja.size(), nba};
+    out.ade[cls] = {append_table(b->h_jobs, ja.data(), ja.size() * sizeof(AdeJob<T>)), (int)ja.size(), nba};
   }
   std::vector<DftJob<T>> jd;
   int64_t nbd = 0;
-  for (gmes_sim* s : b->sims) dft_jobs<T>(s, jd, nbd);
-  b->dft = {append_table(b->h_jobs, jd.data(), jd.size() * sizeof(DftJob<T>)), (int)jd.size(), nbd};
+  for (gmes_sim* s : sims) dft_jobs<T>(s, jd, nbd);
+  out.dft = {append_table(b->h_jobs, jd.data(), jd.size() * sizeof(DftJob<T>)), (int)jd.size(), nbd};
 }
 
 template <class J>
@@ -1732,33 +1737,43 @@ const J* job_ptr(const gmes_batch* b, size_t off) {
   return reinterpret_cast<const J*>(static_cast<const char*>(b->d_jobs) + off);
 }
 
-// One step of every member, in the single-sim order, with the main, source
-// and ghost kernels of all members batched into one launch each.
+// One step of a group of sims (the members, or their auxiliary grids)
+// with one launch per kernel kind.
 template <class T>
-void enqueue_batch_step_t(gmes_batch* b, hipStream_t st) {
-  const auto& H = b->ghost[1];
+void enqueue_set_step_t(gmes_batch* b, const gmes_batch::JobSet& js, const std::vector<gmes_sim*>& sims,
+                        bool with_aux, hipStream_t st) {
+  const auto& H = js.ghost[1];
   launch_ghost_batch<T>(job_ptr<GhostJob<T>>(b, H.off), H.njobs, H.nblocks, 1, st);
   for (int e = 0; e < 2; ++e) {
     const bool electric = e == 0;
     if (!electric) {
-      const auto& G = b->ghost[0];
+      const auto& G = js.ghost[0];
       launch_ghost_batch<T>(job_ptr<GhostJob<T>>(b, G.off), G.njobs, G.nblocks, 0, st);
     }
-    launch_main_batch<T>(electric, kBatch3d, job_ptr<MainJob<T>>(b, b->main3d[e].off), b->main3d[e].njobs,
-                         b->main3d[e].nblocks, st);
-    launch_main_batch<T>(electric, kBatch2z, job_ptr<MainJob<T>>(b, b->main2z[e].off), b->main2z[e].njobs,
-                         b->main2z[e].nblocks, st);
-    launch_main_batch<T>(electric, kBatchGeneric, job_ptr<MainJob<T>>(b, b->mainnd[e].off), b->mainnd[e].njobs,
-                         b->mainnd[e].nblocks, st);
+    launch_main_batch<T>(electric, kBatch3d, job_ptr<MainJob<T>>(b, js.main3d[e].off), js.main3d[e].njobs,
+                         js.main3d[e].nblocks, st);
+    launch_main_batch<T>(electric, kBatch2z, job_ptr<MainJob<T>>(b, js.main2z[e].off), js.main2z[e].njobs,
+                         js.main2z[e].nblocks, st);
+    launch_main_batch<T>(electric, kBatchGeneric, job_ptr<MainJob<T>>(b, js.mainnd[e].off), js.mainnd[e].njobs,
+                         js.mainnd[e].nblocks, st);
     if (electric)
       for (int cls = GMES_CLS_DRUDE; cls < GMES_NCLS; ++cls)
-        launch_ade_batch<T>(cls, job_ptr<AdeJob<T>>(b, b->ade[cls].off), b->ade[cls].njobs, b->ade[cls].nblocks, st);
-    launch_src_batch<T>(job_ptr<SrcJob<T>>(b, b->src[e].off), b->src[e].njobs, b->src[e].nblocks, st);
-    for (gmes_sim* s : b->sims) enqueue_probe<T>(s, electric, st);
-    if (electric)
-      for (gmes_sim* s : b->sims) enqueue_aux_steps(s, st);
+        launch_ade_batch<T>(cls, job_ptr<AdeJob<T>>(b, js.ade[cls].off), js.ade[cls].njobs, js.ade[cls].nblocks, st);
+    launch_src_batch<T>(job_ptr<SrcJob<T>>(b, js.src[e].off), js.src[e].njobs, js.src[e].nblocks, st);
+    for (gmes_sim* s : sims) enqueue_probe<T>(s, electric, st);
+    // every member's auxiliary grids advance one whole step here, all of
+    // them in one launch per kernel kind (as each single sim does it with
+    // its own grids)
+    if (electric && with_aux && !b->aux_sims.empty()) enqueue_set_step_t<T>(b, b->aux_set, b->aux_sims, false, st);
   }
-  launch_dft_batch<T>(job_ptr<DftJob<T>>(b, b->dft.off), b->dft.njobs, b->dft.nblocks, st);
+  launch_dft_batch<T>(job_ptr<DftJob<T>>(b, js.dft.off), js.dft.njobs, js.dft.nblocks, st);
+}
+
+// One step of every member, in the single-sim order, with the main, source
+// and ghost kernels of all members batched into one launch each.
+template <class T>
+void enqueue_batch_step_t(gmes_batch* b, hipStream_t st) {
+  enqueue_set_step_t<T>(b, b->set, b->sims, true, st);
 }
 
 void enqueue_batch_step(gmes_batch* b, hipStream_t st) {
@@ -1856,11 +1871,22 @@ int gmes_batch_step(gmes_batch* b, int64_t nsteps) {
   HIP_TRY(hipSetDevice(b->device));
   // the job tables are read by kernels still in flight: wait before rewriting
   HIP_TRY(hipStreamSynchronize(b->stream));
+  b->aux_sims.clear();
+  for (gmes_sim* s : b->sims)
+    for (int i = 0; i < s->naux; ++i)
+      if (s->aux[i]) b->aux_sims.push_back(s->aux[i]);
+  b->h_jobs.clear();
   switch (b->dtype) {
-    case GMES_F32: build_jobs<float>(b); break;
-    case GMES_F64: build_jobs<double>(b); break;
-    case GMES_C64: build_jobs<cplx<float>>(b); break;
-    default: build_jobs<cplx<double>>(b); break;
+    case GMES_F32: build_jobs<float>(b, b->sims, b->set); build_jobs<float>(b, b->aux_sims, b->aux_set); break;
+    case GMES_F64: build_jobs<double>(b, b->sims, b->set); build_jobs<double>(b, b->aux_sims, b->aux_set); break;
+    case GMES_C64:
+      build_jobs<cplx<float>>(b, b->sims, b->set);
+      build_jobs<cplx<float>>(b, b->aux_sims, b->aux_set);
+      break;
+    default:
+      build_jobs<cplx<double>>(b, b->sims, b->set);
+      build_jobs<cplx<double>>(b, b->aux_sims, b->aux_set);
+      break;
   }
   if (b->h_jobs.size() > b->jobs_cap) {
     destroy_batch_graph(b);
@@ -1874,20 +1900,22 @@ int gmes_batch_step(gmes_batch* b, int64_t nsteps) {
   // launch shapes and per-member captured arguments decide whether the
   // graph is still valid; table contents are read at run time
   std::vector<int64_t> sig;
-  for (int e = 0; e < 2; ++e)
-    for (const auto* L : {&b->main3d[e], &b->main2z[e], &b->mainnd[e], &b->src[e], &b->ghost[e]}) {
-      sig.push_back((int64_t)L->off);
-      sig.push_back(L->njobs);
-      sig.push_back(L->nblocks);
+  for (const gmes_batch::JobSet* js : {&b->set, &b->aux_set}) {
+    for (int e = 0; e < 2; ++e)
+      for (const auto* L : {&js->main3d[e], &js->main2z[e], &js->mainnd[e], &js->src[e], &js->ghost[e]}) {
+        sig.push_back((int64_t)L->off);
+        sig.push_back(L->njobs);
+        sig.push_back(L->nblocks);
+      }
+    for (const auto& L : js->ade) {
+      sig.push_back((int64_t)L.off);
+      sig.push_back(L.njobs);
+      sig.push_back(L.nblocks);
     }
-  for (const auto& L : b->ade) {
-    sig.push_back((int64_t)L.off);
-    sig.push_back(L.njobs);
-    sig.push_back(L.nblocks);
+    sig.push_back((int64_t)js->dft.off);
+    sig.push_back(js->dft.njobs);
+    sig.push_back(js->dft.nblocks);
   }
-  sig.push_back((int64_t)b->dft.off);
-  sig.push_back(b->dft.njobs);
-  sig.push_back(b->dft.nblocks);
   std::vector<uint64_t> gens;
   for (gmes_sim* s : b->sims) {
     gens.push_back(s->gen);

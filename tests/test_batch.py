@@ -9,7 +9,7 @@ import pytest
 
 import gmes_amd as g
 from gmes_amd.batch import Batch
-from cases import (bloch2d, bloch3d, const_case, dispersive2d, drude, fields, kat_tmz, lorentz,
+from cases import (bloch2d, bloch3d, const_case, dispersive2d, drude, fields, kat_tmz, lorentz, tfsf3d,
                    mode_case, rel_l2, spheres3d, tfsf2d)
 
 
@@ -24,6 +24,7 @@ def real_members(lib, precision):
         dispersive2d(lorentz(), lib, precision=precision, mode=g.TEzFDTD),
         const_case(lib, precision=precision),
         tfsf2d(lib, precision=precision, res=10),
+        tfsf2d(lib, precision=precision, res=10, scatterer=False),  # a second auxiliary grid
     ]
 
 
@@ -32,6 +33,7 @@ def complex_members(lib, precision):
         bloch2d(lib, precision=precision),
         bloch3d(lib, precision=precision),
         tfsf2d(lib, precision=precision, res=10, bloch=(0, 0, 0)),
+        tfsf3d(lib, precision=precision, bloch=(0.3, 0, 0.1)),
         dispersive2d(drude(), lib, precision=precision, bloch=(0.2, 0.1, 0)),
     ]
 
