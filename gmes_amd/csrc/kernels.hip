@@ -1102,13 +1102,14 @@ __global__ void __launch_bounds__(256) k_dft_batch(const DftJob<T>* jobs, int nj
   const int k = (int)(q % J.n[2]);
   const int j = (int)((q / J.n[2]) % J.n[1]);
   const int i = (int)(q / ((int64_t)J.n[2] * J.n[1]));
-  const T v = J.f[kPad + ((i + J.lo[0]) * J.s1 + (j + J.lo[1])) * J.p + (k + J.lo[2]) - J.koff];
+  const T v = gld(J.f + (kPad + ((i + J.lo[0]) * J.s1 + (j + J.lo[1])) * J.p + (k + J.lo[2]) - J.koff));
   const double vr = (double)re_of(v), vi = (double)im_of(v);
-  const double* w = J.w + row * J.nfreq * 4 + (J.electric ? 0 : 2);
-  double* acc = J.acc + 2 * q;
+  // the job table's pointers are global memory: address them as such
+  const GMES_GLOBAL double* w = (const GMES_GLOBAL double*)J.w + row * J.nfreq * 4 + (J.electric ? 0 : 2);
+  GMES_GLOBAL double* acc = (GMES_GLOBAL double*)J.acc + 2 * q;
   for (int f = 0; f < J.nfreq; ++f) {
     const double wr = w[4 * f], wi = w[4 * f + 1];
-    double* a = acc + 2 * f * J.ncells;
+    GMES_GLOBAL double* a = acc + 2 * f * J.ncells;
     a[0] = a[0] + (wr * vr - wi * vi);
     a[1] = a[1] + (wr * vi + wi * vr);
   }
