@@ -245,6 +245,9 @@ int upload_tables(gmes_sim* s) {
         for (size_t q = 0; q < src.size(); ++q)
           pr.push_back((R)(recip && q % 3 == 2 ? 1.0 / src[q] : src[q]));
       }
+  if (pr.size() > (size_t)INT32_MAX) return set_error(GMES_EINVAL, "profile pool too large");
+  for (int c = 0; c < 6; ++c)
+    for (int a = 0; a < 3; ++a) s->prof_off0[c][a] = off[c * 3 + a];
   if (pr.empty()) pr.push_back(R(0));
   if (cf.empty()) cf.push_back(R(0));
   int rc;
@@ -393,6 +396,8 @@ bool main_args(const gmes_sim* s, bool electric, MainArgs<T>& m, int ext[3]) {
   for (int c = 0; c < 6; ++c)
     if (s->pml_state[c] && s->has_cls[c][GMES_CLS_CPML]) m.cpml_mask |= 1u << c;
   for (int a = 0; a < 3; ++a) m.prun[a] = s->prun[a];
+  for (int c = 0; c < 6; ++c)
+    for (int a = 0; a < 3; ++a) m.po0[c][a] = (int)s->prof_off0[c][a];
   m.narrow = 1;
   for (int c = 0; c < 6; ++c) {
     m.ids[c] = s->ids[c];

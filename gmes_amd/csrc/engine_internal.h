@@ -84,6 +84,7 @@ struct gmes_sim {
   uint16_t* ids[6] = {};     // device ids
   uint8_t* ids8[6] = {};     // byte copies for the 3-D main kernels (at most 256 records)
   int32_t prun[3] = {};      // per axis: run (lo | hi << 16) where every CPML profile's c is zero
+  int64_t prof_off0[6][3] = {};  // pool offsets of profile set 0 (component, axis)
   std::vector<uint16_t> host_ids[6];
   std::vector<gmes_mat_rec> recs;
   std::vector<double> coef;

@@ -70,6 +70,7 @@ struct MainArgs {
   // the coordinate run (lo | hi << 16) where every CPML profile's c is zero
   unsigned cpml_mask;
   int prun[3];
+  int po0[6][3];  // offsets of profile set 0's rows (component, axis) in the pool
 };
 
 template <class T>

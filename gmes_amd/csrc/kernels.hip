@@ -197,10 +197,12 @@ __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typ
     const T a = vscl(id1, D1), b = vscl(id2, D2);  // D1/d1, D2/d2
     if (cls == GMES_CLS_DIELECTRIC) return E ? fma_(rec.c0, vsub(a, b), f) : fma_(rec.c0, vsub(b, a), f);
     const int ix[3] = {i, j, k};
+    // profile rows of set 0 come from the arguments (no dependent load)
     const GMES_GLOBAL int64_t* po = (const GMES_GLOBAL int64_t*)m.prof_off + (rec.prof * 6 + C) * 3;
+    const bool set0 = rec.prof == 0;
     if (cls == GMES_CLS_CPML) {
-      const GMES_GLOBAL float* p1 = (const GMES_GLOBAL float*)m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 3;
-      const GMES_GLOBAL float* p2 = (const GMES_GLOBAL float*)m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 3;
+      const GMES_GLOBAL float* p1 = (const GMES_GLOBAL float*)m.prof + (set0 ? (int64_t)m.po0[C][Ax<C>::a1] : po[Ax<C>::a1]) + ix[Ax<C>::a1] * 3;
+      const GMES_GLOBAL float* p2 = (const GMES_GLOBAL float*)m.prof + (set0 ? (int64_t)m.po0[C][Ax<C>::a2] : po[Ax<C>::a2]) + ix[Ax<C>::a2] * 3;
       const bool u1 = psi_live<Ax<C>::a1>(rec, ix), u2 = psi_live<Ax<C>::a2>(rec, ix);
       T psi1, psi2;
       if constexpr (sizeof(T) >= 8) {
@@ -226,9 +228,9 @@ __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typ
       return E ? fma_(rec.c0, t, f) : fma_(-rec.c0, t, f);
     }
     // UPML
-    const GMES_GLOBAL float* p1 = (const GMES_GLOBAL float*)m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 6;
-    const GMES_GLOBAL float* p2 = (const GMES_GLOBAL float*)m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 6;
-    const GMES_GLOBAL float* p0 = (const GMES_GLOBAL float*)m.prof + po[Ax<C>::a0] + ix[Ax<C>::a0] * 6;
+    const GMES_GLOBAL float* p1 = (const GMES_GLOBAL float*)m.prof + (set0 ? (int64_t)m.po0[C][Ax<C>::a1] : po[Ax<C>::a1]) + ix[Ax<C>::a1] * 6;
+    const GMES_GLOBAL float* p2 = (const GMES_GLOBAL float*)m.prof + (set0 ? (int64_t)m.po0[C][Ax<C>::a2] : po[Ax<C>::a2]) + ix[Ax<C>::a2] * 6;
+    const GMES_GLOBAL float* p0 = (const GMES_GLOBAL float*)m.prof + (set0 ? (int64_t)m.po0[C][Ax<C>::a0] : po[Ax<C>::a0]) + ix[Ax<C>::a0] * 6;
     const T dstore = gld(st + flat);
     const T cu = vsub(a, b);
     const T dn = E ? fma_(p1[1], cu, vscl(p1[0], dstore)) : fma_(-p1[1], cu, vscl(p1[0], dstore));
@@ -241,10 +243,12 @@ __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typ
       else return vadd(f, vscl(rec.c0, vsub(vdiv(D2, d2), vdiv(D1, d1))));
     }
     const int ix[3] = {i, j, k};
+    // profile rows of set 0 come from the arguments (no dependent load)
     const GMES_GLOBAL int64_t* po = (const GMES_GLOBAL int64_t*)m.prof_off + (rec.prof * 6 + C) * 3;
+    const bool set0 = rec.prof == 0;
     if (cls == GMES_CLS_CPML) {
-      const GMES_GLOBAL R* p1 = (const GMES_GLOBAL R*)m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 3;
-      const GMES_GLOBAL R* p2 = (const GMES_GLOBAL R*)m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 3;
+      const GMES_GLOBAL R* p1 = (const GMES_GLOBAL R*)m.prof + (set0 ? (int64_t)m.po0[C][Ax<C>::a1] : po[Ax<C>::a1]) + ix[Ax<C>::a1] * 3;
+      const GMES_GLOBAL R* p2 = (const GMES_GLOBAL R*)m.prof + (set0 ? (int64_t)m.po0[C][Ax<C>::a2] : po[Ax<C>::a2]) + ix[Ax<C>::a2] * 3;
       const bool u1 = psi_live<Ax<C>::a1>(rec, ix), u2 = psi_live<Ax<C>::a2>(rec, ix);
       const T o1 = u1 ? gld(st + flat) : vzero<T>();
       const T o2 = u2 ? gld(st + m.dsize[C] + flat) : vzero<T>();
@@ -257,9 +261,9 @@ __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typ
       else return vsub(f, vscl(rec.c0, t));
     }
     // UPML
-    const GMES_GLOBAL R* p1 = (const GMES_GLOBAL R*)m.prof + po[Ax<C>::a1] + ix[Ax<C>::a1] * 6;
-    const GMES_GLOBAL R* p2 = (const GMES_GLOBAL R*)m.prof + po[Ax<C>::a2] + ix[Ax<C>::a2] * 6;
-    const GMES_GLOBAL R* p0 = (const GMES_GLOBAL R*)m.prof + po[Ax<C>::a0] + ix[Ax<C>::a0] * 6;
+    const GMES_GLOBAL R* p1 = (const GMES_GLOBAL R*)m.prof + (set0 ? (int64_t)m.po0[C][Ax<C>::a1] : po[Ax<C>::a1]) + ix[Ax<C>::a1] * 6;
+    const GMES_GLOBAL R* p2 = (const GMES_GLOBAL R*)m.prof + (set0 ? (int64_t)m.po0[C][Ax<C>::a2] : po[Ax<C>::a2]) + ix[Ax<C>::a2] * 6;
+    const GMES_GLOBAL R* p0 = (const GMES_GLOBAL R*)m.prof + (set0 ? (int64_t)m.po0[C][Ax<C>::a0] : po[Ax<C>::a0]) + ix[Ax<C>::a0] * 6;
     const T dstore = gld(st + flat);
     const T cu = vsub(vdiv(D1, d1), vdiv(D2, d2));
     T dn;
