@@ -208,6 +208,56 @@ int32_t zero_c_run(const gmes_sim* s, int p, int axis) {
   return lo | (hi << 16);
 }
 
+// Real-FP32 3-D sims: the components with UPML (and no CPML) state whose
+// state word the 3-D kernels load together with the cell's other operands,
+// and a box of storage coordinates (per axis lo | hi << 16) holding no UPML
+// cell of those components, inside which the kernels read the array's
+// guard word instead (its value is unused there).  The box is the run of
+// non-UPML cells through each component's centre along each axis,
+// intersected over the components, and is kept only if no cell inside it
+// is UPML; otherwise the kernels load the word after the cell's record.
+void upml_box(gmes_sim* s) {
+  s->upml_pre = 0;
+  for (int a = 0; a < 3; ++a) s->ubox[a] = 0;
+  if (!is3d(s) || s->g.dtype != GMES_F32) return;
+  int64_t lo[3] = {0, 0, 0}, hi[3] = {0x7FFF, 0x7FFF, 0x7FFF};
+  unsigned mask = 0;
+  for (int c = 0; c < 6; ++c) {
+    if (!active(s, c) || !s->pml_state[c] || s->has_cls[c][GMES_CLS_CPML] || !s->has_cls[c][GMES_CLS_UPML]) continue;
+    mask |= 1u << c;
+    const int64_t* n = s->shape[c];
+    const std::vector<uint16_t>& ids = s->host_ids[c];
+    for (int a = 0; a < 3; ++a) {
+      int64_t ix[3] = {n[0] / 2, n[1] / 2, n[2] / 2};
+      auto upml_at = [&](int64_t v) {
+        ix[a] = v;
+        return s->recs[ids[(ix[0] * n[1] + ix[1]) * n[2] + ix[2]]].cls == GMES_CLS_UPML;
+      };
+      const int64_t mid = n[a] / 2;
+      if (upml_at(mid)) return;
+      int64_t l = mid, h = mid + 1;
+      while (l > 0 && !upml_at(l - 1)) --l;
+      while (h < n[a] && !upml_at(h)) ++h;
+      lo[a] = std::max(lo[a], l);
+      hi[a] = std::min(hi[a], h);
+    }
+  }
+  if (!mask) return;
+  for (int a = 0; a < 3; ++a)
+    if (lo[a] >= hi[a]) return;
+  for (int c = 0; c < 6; ++c) {
+    if (!((mask >> c) & 1u)) continue;
+    const int64_t* n = s->shape[c];
+    const std::vector<uint16_t>& ids = s->host_ids[c];
+    for (int64_t i = lo[0]; i < std::min(hi[0], n[0]); ++i)
+      for (int64_t j = lo[1]; j < std::min(hi[1], n[1]); ++j)
+        for (int64_t k = lo[2]; k < std::min(hi[2], n[2]); ++k)
+          if (s->recs[ids[(i * n[1] + j) * n[2] + k]].cls == GMES_CLS_UPML) return;
+  }
+  s->upml_pre = mask;
+  for (int a = 0; a < 3; ++a) s->ubox[a] = (int32_t)(lo[a] | (hi[a] << 16));
+}
+
 template <class R>
 int upload_tables(gmes_sim* s) {
   std::vector<DevRec<R>> dr(s->recs.size());
@@ -396,6 +446,8 @@ bool main_args(const gmes_sim* s, bool electric, MainArgs<T>& m, int ext[3]) {
   for (int c = 0; c < 6; ++c)
     if (s->pml_state[c] && s->has_cls[c][GMES_CLS_CPML]) m.cpml_mask |= 1u << c;
   for (int a = 0; a < 3; ++a) m.prun[a] = s->prun[a];
+  m.upml_mask = s->upml_pre;
+  for (int a = 0; a < 3; ++a) m.ubox[a] = s->ubox[a];
   for (int c = 0; c < 6; ++c)
     for (int a = 0; a < 3; ++a) m.po0[c][a] = (int)s->prof_off0[c][a];
   m.narrow = 1;
@@ -1212,6 +1264,7 @@ int gmes_sim_commit(gmes_sim* s) {
     }
     s->prun[a] = (any && lo < hi) ? (lo | (hi << 16)) : 0;
   }
+  upml_box(s);
   s->committed = true;
   destroy_graph(s);
   decide_fusion(s);

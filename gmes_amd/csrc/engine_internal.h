@@ -85,6 +85,8 @@ struct gmes_sim {
   uint8_t* ids8[6] = {};     // byte copies for the 3-D main kernels (at most 256 records)
   int32_t prun[3] = {};      // per axis: run (lo | hi << 16) where every CPML profile's c is zero
   int64_t prof_off0[6][3] = {};  // pool offsets of profile set 0 (component, axis)
+  uint32_t upml_pre = 0;     // components whose UPML state the 3-D kernels prefetch (bit c)
+  int32_t ubox[3] = {};      // per axis (lo | hi << 16): a box holding no UPML cell of those
   std::vector<uint16_t> host_ids[6];
   std::vector<gmes_mat_rec> recs;
   std::vector<double> coef;

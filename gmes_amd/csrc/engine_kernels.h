@@ -71,6 +71,11 @@ struct MainArgs {
   unsigned cpml_mask;
   int prun[3];
   int po0[6][3];  // offsets of profile set 0's rows (component, axis) in the pool
+  // real-FP32 3-D kernels: components whose UPML state word is loaded with
+  // the other operands (bit c), and a box (per axis lo | hi << 16) that
+  // holds no UPML cell of those components
+  unsigned upml_mask;
+  int ubox[3];
 };
 
 template <class T>

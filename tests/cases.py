@@ -48,9 +48,9 @@ def air2d(lib, precision="fp64"):
     return sim
 
 
-def spheres3d(lib, precision="fp64", n=24, nsph=6, pml=4, seed=2, courant=0.5):
-    """A reduced 3-D dielectric-spheres box with a CPML on every face and a
-    Gaussian-pulse Ex point source at the centre."""
+def spheres3d(lib, precision="fp64", n=24, nsph=6, pml=4, seed=2, courant=0.5, bound=None):
+    """A reduced 3-D dielectric-spheres box with a CPML (or ``bound``) on
+    every face and a Gaussian-pulse Ex point source at the centre."""
     rng = np.random.default_rng(seed)
     half = n / 2
     space = g.Cartesian(size=(n, n, n), resolution=1)
@@ -60,7 +60,7 @@ def spheres3d(lib, precision="fp64", n=24, nsph=6, pml=4, seed=2, courant=0.5):
         lim = half - pml - r
         c = rng.uniform(-lim, lim, 3)
         geometry.append(g.Sphere(g.Dielectric(rng.uniform(1.5, 12)), center=tuple(c), radius=r))
-    geometry.append(g.Shell(g.Cpml(), thickness=pml))
+    geometry.append(g.Shell((bound or g.Cpml)(), thickness=pml))
     src = [g.PointSource(g.Bandpass(freq=0.05, fwidth=0.025), center=(0, 0, 0), component=g.Ex)]
     sim = g.FDTD(space, geometry, src, courant_ratio=courant, verbose=False, precision=precision,
                  _engine=lib)

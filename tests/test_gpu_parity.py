@@ -109,6 +109,18 @@ def test_const_and_upml3d(engine_lib, oracle_lib):
     assert rel_l2(gpu, ref) <= FP64_TOL
 
 
+@pytest.mark.parametrize("builder,kw", [(spheres3d, dict(bound=g.Upml)), (const_case, {})],
+                         ids=["spheres3d_upml", "const_upml"])
+def test_upml3d_fp32(engine_lib, oracle_lib, builder, kw):
+    """3-D UPML boxes: FP64 equals the oracle; FP32, whose 3-D kernels load
+    the UPML state word with the cell's other operands, stays within 1e-5
+    of the FP64 run."""
+    steps = 40
+    gpu = run(builder, engine_lib, steps, **kw)
+    assert rel_l2(gpu, run(builder, oracle_lib, steps, **kw)) <= FP64_TOL
+    assert rel_l2(run(builder, engine_lib, steps, precision="fp32", **kw), gpu) <= 1e-5
+
+
 def test_dm2(engine_lib, oracle_lib):
     gpu = run(dm2_1d, engine_lib, 80)
     ref = run(dm2_1d, oracle_lib, 80)
