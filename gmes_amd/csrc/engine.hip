@@ -406,7 +406,13 @@ GhostArgs<T> ghost_args(const gmes_sim* s, bool electric, int tick, int64_t& tot
     const int a1 = kGhostOrder[c][0], a2 = kGhostOrder[c][1], a0 = c % 3;
     g.comp[t] = c;
     g.ax[t][0] = a1; g.ax[t][1] = a2; g.ax[t][2] = a0;
-    g.count[t] = s->shape[c][a0] * s->shape[c][a2] + s->shape[c][a0] * (s->shape[c][a1] - 1);
+    const int64_t p1 = s->shape[c][a0] * s->shape[c][a2], p2 = s->shape[c][a0] * (s->shape[c][a1] - 1);
+    g.count[t] = p1 + p2;
+    g.skip[t] = 0;
+    if (!electric && s->fuse_hz && s->shape[c][2] == 2) {  // the z plane is the E kernel's
+      if (a2 == 2) g.count[t] = p1;
+      else if (a1 == 2) { g.skip[t] = p1; g.count[t] = p2; }
+    }
     g.ph[t][0] = cplx<R>{(R)s->g.phase[c][a1][0], (R)s->g.phase[c][a1][1]};
     g.ph[t][1] = cplx<R>{(R)s->g.phase[c][a2][0], (R)s->g.phase[c][a2][1]};
     total += g.count[t];
@@ -448,6 +454,12 @@ bool main_args(const gmes_sim* s, bool electric, MainArgs<T>& m, int ext[3]) {
   for (int a = 0; a < 3; ++a) m.prun[a] = s->prun[a];
   m.upml_mask = s->upml_pre;
   for (int a = 0; a < 3; ++a) m.ubox[a] = s->ubox[a];
+  m.hzg = electric && s->fuse_hz;
+  if (m.hzg) {
+    m.hzph[0] = cplx<R>{(R)s->g.phase[3][2][0], (R)s->g.phase[3][2][1]};
+    m.hzph[1] = cplx<R>{(R)s->g.phase[4][2][0], (R)s->g.phase[4][2][1]};
+    m.hyxph = cplx<R>{(R)s->g.phase[4][0][0], (R)s->g.phase[4][0][1]};
+  }
   for (int c = 0; c < 6; ++c)
     for (int a = 0; a < 3; ++a) m.po0[c][a] = (int)s->prof_off0[c][a];
   m.narrow = 1;
@@ -929,15 +941,30 @@ bool on_ghost_copy_plane(const gmes_sim* s, int c, int64_t x, bool source, bool 
   return false;
 }
 
+//
+// In a 2-D mode with a collapsed z axis where Ez is the only E component
+// (TMz), the z ghost plane (k = 0) of Hx and Hy is read by no E update
+// (Ez reads k = 1), and nothing runs between the H ghost copy and the E
+// kernel; the 2-D E kernel then writes that plane itself from k = 1, with
+// the copy's phases and corner order, and the H copy leaves it out.
 void decide_fusion(gmes_sim* s) {
-  const bool prev = s->fuse_e;
+  const bool prev = s->fuse_e, prev_hz = s->fuse_hz;
   bool f = s->allow_fuse && is3d(s) && s->committed && !s->ade_on_ghost_source;
   for (size_t n = 0; f && n < s->ops.size(); ++n)
     if (s->ops[n].comp < 3 && on_ghost_copy_plane(s, s->ops[n].comp, s->ops[n].cell, true, true)) f = false;
   for (size_t n = 0; f && n < s->probes.size(); ++n)
     if (s->probes[n].comp < 3 && on_ghost_copy_plane(s, s->probes[n].comp, s->probes[n].cell, false, true)) f = false;
   s->fuse_e = f;
-  if (f != prev) destroy_graph(s);
+  bool hz = s->allow_fuse && s->committed && is_2z(s) && active(s, 2) && !active(s, 0) && !active(s, 1);
+  bool any = false;
+  for (int c = 3; hz && c < 5; ++c) {
+    if (!active(s, c) || s->shape[c][2] != 2) continue;
+    any = true;
+    // the E kernel's threads span Ez's (i, j) extent
+    if (s->shape[c][0] > s->shape[2][0] || s->shape[c][1] > s->shape[2][1]) hz = false;
+  }
+  s->fuse_hz = hz && any;
+  if (f != prev || s->fuse_hz != prev_hz) destroy_graph(s);
 }
 
 int ctl_write(gmes_sim* s) {
@@ -1604,7 +1631,7 @@ int gmes_sim_set_ghost_fusion(gmes_sim* s, int32_t enable, int32_t* fused) {
   HIP_TRY(sync_exec(s));
   s->allow_fuse = enable != 0;
   decide_fusion(s);
-  if (fused) *fused = s->fuse_e ? 1 : 0;
+  if (fused) *fused = (s->fuse_e || s->fuse_hz) ? 1 : 0;
   return GMES_OK;
 }
 

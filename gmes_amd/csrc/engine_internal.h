@@ -120,6 +120,8 @@ struct gmes_sim {
   // ade_on_ghost_source: a dispersive E cell lies on plane 0 of a
   // transverse axis (set by commit), which rules that out
   bool fuse_e = false, ade_on_ghost_source = false, allow_fuse = true;
+  // fuse_hz: the 2-D (TMz) E kernel writes the z ghost plane of Hx and Hy
+  bool fuse_hz = false;
   bool aux_lost = false;  // an attached auxiliary grid was destroyed
   uint64_t gen = 0;            // bumped whenever captured launch arguments go stale
   void* d_aux_ptrs = nullptr;  // [8][6] device pointers of aux fields

@@ -76,6 +76,13 @@ struct MainArgs {
   // holds no UPML cell of those components
   unsigned upml_mask;
   int ubox[3];
+  // 2-D modes with a collapsed z axis (TMz): 1 when the E kernel writes the
+  // z ghost plane (k = 0) of Hx and Hy, which no E update reads, in place
+  // of that part of the H ghost copy; hzph: the copy's phase along z for
+  // Hx and Hy; hyxph: Hy's phase along x (its corner line)
+  int hzg;
+  cplx<R> hzph[2];
+  cplx<R> hyxph;
 };
 
 template <class T>
@@ -128,6 +135,7 @@ struct GhostArgs {
   cplx<R> ph[3][2];  // phase of the copy along a1, a2
   int tick;
   Ctl* ctl;
+  int64_t skip[3];   // leading points of a component's copy left out (written elsewhere)
 };
 
 // Batched-launch job tables: one entry per member sim; block0 is the

@@ -469,6 +469,30 @@ __device__ __forceinline__ void update2z(const MainArgs<T>& m, const DevRec<type
   }
 }
 
+// TMz: the z ghost plane (k = 0) of Hx and Hy, which no E update reads,
+// written by the E kernel from k = 1 in place of that part of the H ghost
+// copy (k_ghost), which ran just before.  Same values as the copy: Hx's
+// corner line with its y ghost plane (j = 0) stays in the copy; Hy's with
+// its x ghost plane (i = 0) is phx * (phz * Hy(last, j, 1)), the copy's
+// order (z first, then x).
+template <class T, int N>
+__device__ __forceinline__ void h_zghost(const MainArgs<T>& m, int i0, int j) {
+  const FieldArgs<T>& a = m.fa;
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    const int i = i0 + n;
+    if (((a.active >> 3) & 1u) && a.s[3][2] == 2 && i < a.s[3][0] && j >= 1 && j < a.s[3][1]) {
+      const T v = gld(a.f[3] + didx(a, 3, i, j, 1));
+      gst(a.f[3] + didx(a, 3, i, j, 0), ghost_phase<T>(m.hzph[0], v));
+    }
+    if (((a.active >> 4) & 1u) && a.s[4][2] == 2 && i < a.s[4][0] && j < a.s[4][1]) {
+      T v = ghost_phase<T>(m.hzph[1], gld(a.f[4] + didx(a, 4, i == 0 ? a.s[4][0] - 1 : i, j, 1)));
+      if (i == 0) v = ghost_phase<T>(m.hyxph, v);
+      gst(a.f[4] + didx(a, 4, i, j, 0), v);
+    }
+  }
+}
+
 template <bool E, class T, int N>
 __device__ __forceinline__ void main2z_tile(const MainArgs<T>& m, DevRec<typename real_of<T>::type>* srec, int i0,
                                             int j, int tid, int nthreads) {
@@ -489,6 +513,9 @@ __device__ __forceinline__ void main2z_tile(const MainArgs<T>& m, DevRec<typenam
     update2z<C0 + 0, T, N>(m, m.recs, q, i0, j);
     update2z<C0 + 1, T, N>(m, m.recs, q, i0, j);
     update2z<C0 + 2, T, N>(m, m.recs, q, i0, j);
+  }
+  if constexpr (E) {
+    if (m.hzg) h_zghost<T, N>(m, i0, j);
   }
 }
 
@@ -1216,7 +1243,7 @@ __device__ __forceinline__ void ghost_point(const GhostArgs<T>& m, int64_t e) {
   int t = 0;
   while (t < m.ncomp && e >= m.count[t]) { e -= m.count[t]; ++t; }
   if (t >= m.ncomp) return;
-  int r = (int)e;  // a point of one component: 32-bit index arithmetic
+  int r = (int)(e + m.skip[t]);  // a point of one component: 32-bit index arithmetic
   const int c = m.comp[t];
   const int a1 = m.ax[t][0], a2 = m.ax[t][1], a0 = m.ax[t][2];
   const int* s = m.fa.s[c];

@@ -232,13 +232,16 @@ int gmes_sim_set_tile(gmes_sim* sim, int32_t bx, int32_t by);
  * the chosen tile.  No reference counterpart: the reference's update loops
  * have no launch shape (src/cpp23_support.hh:147-197). */
 int gmes_sim_autotune(gmes_sim* sim, int32_t* bx, int32_t* by);
-/* Fused E ghost planes.  enable = 1 (the default) lets the 3-D E kernel
+/* Fused ghost planes.  enable = 1 (the default) lets the 3-D E kernel
  * write the periodic / Bloch E ghost planes itself, in place of the copy
- * that follows the E sources (gmes/fdtd.py:955-1097), whenever that gives
+ * that follows the E sources (gmes/fdtd.py:811-953), whenever that gives
  * the same fields: no dispersive E cell on a source plane, no E source
- * operation on a source or ghost plane, no E probe on a ghost plane.
- * enable = 0 keeps the separate copy.  *fused (may be NULL) reports
- * whether the fusion is in effect now.  No reference counterpart. */
+ * operation on a source or ghost plane, no E probe on a ghost plane.  In
+ * TMz it lets the 2-D E kernel write the z ghost plane of Hx and Hy (read
+ * by no E update) in place of that part of the H copy at the start of the
+ * step (gmes/fdtd.py:955-1097).  enable = 0 keeps the separate copies.
+ * *fused (may be NULL) reports whether either fusion is in effect now.
+ * No reference counterpart. */
 int gmes_sim_set_ghost_fusion(gmes_sim* sim, int32_t enable, int32_t* fused);
 /* Kernel timing on the sim's stream: enable accumulates HIP-event time per
  * kernel family; read returns ms and launch counts for

@@ -268,6 +268,26 @@ def test_fused_e_ghosts_match_separate_copy(engine_lib, builder, kw):
             np.testing.assert_array_equal(fa[c], fb[c])
 
 
+@pytest.mark.parametrize("builder,kw", [(kat_tmz, {}), (bloch2d, {}), (bloch2d, dict(precision="fp32")),
+                                        (tfsf2d, dict(res=10, bloch=(0, 0, 0))),
+                                        (lambda lib, **kw: dispersive2d(drude(), lib, **kw), {})],
+                         ids=["kat_tmz", "bloch2d", "bloch2d_c64", "tfsf2d_bloch", "drude_tmz"])
+def test_fused_h_z_ghosts_match_separate_copy(engine_lib, builder, kw):
+    """TMz: the z ghost plane of Hx and Hy written by the 2-D E kernel (from
+    k = 1, Hy's corner line with both phases) gives the same fields, bit for
+    bit, as the separate H ghost copy at the start of the step."""
+    a, b = builder(engine_lib, **kw), builder(engine_lib, **kw)
+    assert engine_lib.set_ghost_fusion(a._sim, 1)
+    assert not engine_lib.set_ghost_fusion(b._sim, 0)
+    for _ in range(3):
+        a._advance(11)
+        b._advance(11)
+        fa, fb = fields(a), fields(b)
+        assert np.abs(fa[g.Hy][:, :, 0]).max() > 0
+        for c in fa:
+            np.testing.assert_array_equal(fa[c], fb[c])
+
+
 def test_ghost_fusion_off_for_dispersive_source_planes(engine_lib):
     """The dispersive Bloch cell's DCP slab covers the x and y source planes:
     its E ghosts keep the separate copy."""
