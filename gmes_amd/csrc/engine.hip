@@ -924,9 +924,9 @@ int autotune_t(gmes_sim* s, int& best_bx, int& best_by) {
   return GMES_OK;
 }
 
-// Whether the 3-D E kernel may write the E ghost planes itself instead of
-// the separate copy that runs later in the step (after the dispersive
-// updates, the E sources and the E probes).  The results are the same when
+// Whether the 3-D (or 2-D collapsed-z) E kernel may write the E ghost
+// planes itself instead of the separate copy that runs later in the step
+// (after the dispersive updates, the E sources and the E probes).  The results are the same when
 // none of those touches a plane involved in the copy: no dispersive cell
 // on a source plane (plane 0 of a transverse axis), no E source operation
 // on a source or ghost plane, no E probe on a ghost plane (it would read
@@ -949,7 +949,7 @@ bool on_ghost_copy_plane(const gmes_sim* s, int c, int64_t x, bool source, bool 
 // the copy's phases and corner order, and the H copy leaves it out.
 void decide_fusion(gmes_sim* s) {
   const bool prev = s->fuse_e, prev_hz = s->fuse_hz;
-  bool f = s->allow_fuse && is3d(s) && s->committed && !s->ade_on_ghost_source;
+  bool f = s->allow_fuse && (is3d(s) || is_2z(s)) && s->committed && !s->ade_on_ghost_source;
   for (size_t n = 0; f && n < s->ops.size(); ++n)
     if (s->ops[n].comp < 3 && on_ghost_copy_plane(s, s->ops[n].comp, s->ops[n].cell, true, true)) f = false;
   for (size_t n = 0; f && n < s->probes.size(); ++n)

@@ -464,8 +464,14 @@ __device__ __forceinline__ void update2z(const MainArgs<T>& m, const DevRec<type
   for (int n = 0; n < N; ++n) {
     if (!q.upd[c][n]) continue;
     const int flat = q.flat[c][n];
-    (void)store_cell<C>(m, recs[q.id[c][n]], a.f[C], st, flat, i0 + n, j, kc, q.f[c][n], q.d1[c][n],
-                        q.d2[c][n]);
+    const T v = store_cell<C>(m, recs[q.id[c][n]], a.f[C], st, flat, i0 + n, j, kc, q.f[c][n], q.d1[c][n],
+                              q.d2[c][n]);
+    if constexpr (C < 3) {
+      if (m.fuse) {  // E ghost planes (the copy k_ghost<E> would make later)
+        const int ix[3] = {i0 + n, j, kc};
+        fused_ghost<T>(m, C, a.f[C], ix, v);
+      }
+    }
   }
 }
 

@@ -232,9 +232,10 @@ int gmes_sim_set_tile(gmes_sim* sim, int32_t bx, int32_t by);
  * the chosen tile.  No reference counterpart: the reference's update loops
  * have no launch shape (src/cpp23_support.hh:147-197). */
 int gmes_sim_autotune(gmes_sim* sim, int32_t* bx, int32_t* by);
-/* Fused ghost planes.  enable = 1 (the default) lets the 3-D E kernel
- * write the periodic / Bloch E ghost planes itself, in place of the copy
- * that follows the E sources (gmes/fdtd.py:811-953), whenever that gives
+/* Fused ghost planes.  enable = 1 (the default) lets the 3-D and the 2-D
+ * (TMz, TEz) E kernels write the periodic / Bloch E ghost planes
+ * themselves, in place of the copy that follows the E sources
+ * (gmes/fdtd.py:811-953), whenever that gives
  * the same fields: no dispersive E cell on a source plane, no E source
  * operation on a source or ghost plane, no E probe on a ghost plane.  In
  * TMz it lets the 2-D E kernel write the z ghost plane of Hx and Hy (read
