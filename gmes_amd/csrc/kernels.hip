@@ -210,8 +210,8 @@ __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typ
         const T o2 = u2 ? gld(st + m.dsize[C] + flat) : vzero<T>();
         psi1 = fma_(p1[1], a, vscl(p1[0], o1));
         psi2 = fma_(p2[1], b, vscl(p2[0], o2));
-        if (u1) gst(st + flat, psi1);
-        if (u2) gst(st + m.dsize[C] + flat, psi2);
+        if (u1) st_nt(st + flat, psi1);
+        if (u2) st_nt(st + m.dsize[C] + flat, psi2);
       } else {
         // real FP32: branches around the psi words cost more than their
         // traffic (measured).  A dead psi is read from element 0 of its
@@ -221,8 +221,8 @@ __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typ
         const T o2 = PRE ? pre2 : gld(st + m.dsize[C] + (u2 ? flat : 0));
         psi1 = fma_(p1[1], a, vscl(p1[0], o1));
         psi2 = fma_(p2[1], b, vscl(p2[0], o2));
-        gst(st + flat, psi1);
-        gst(st + m.dsize[C] + flat, psi2);
+        st_nt(st + flat, psi1);  // used once per launch, like the field words
+        st_nt(st + m.dsize[C] + flat, psi2);
       }
       const T t = vsub(vadd(fma_(-p2[2], b, vscl(p1[2], a)), psi1), psi2);
       return E ? fma_(rec.c0, t, f) : fma_(-rec.c0, t, f);
@@ -234,7 +234,7 @@ __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typ
     const T dstore = (PRE && ((m.upml_mask >> C) & 1u)) ? pre1 : gld(st + flat);
     const T cu = vsub(a, b);
     const T dn = E ? fma_(p1[1], cu, vscl(p1[0], dstore)) : fma_(-p1[1], cu, vscl(p1[0], dstore));
-    gst(st + flat, dn);
+    st_nt(st + flat, dn);
     return fma_(p2[3] * rec.inv_eps, fma_(-p0[5], dstore, vscl(p0[4], dn)), vscl(p2[2], f));
   } else {
     const R d1 = m.d[Ax<C>::a1], d2 = m.d[Ax<C>::a2];
@@ -254,8 +254,8 @@ __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typ
       const T o2 = u2 ? gld(st + m.dsize[C] + flat) : vzero<T>();
       const T psi1 = vadd(vscl(p1[0], o1), vdiv(vscl(p1[1], D1), d1));
       const T psi2 = vadd(vscl(p2[0], o2), vdiv(vscl(p2[1], D2), d2));
-      if (u1) gst(st + flat, psi1);
-      if (u2) gst(st + m.dsize[C] + flat, psi2);
+      if (u1) st_nt(st + flat, psi1);
+      if (u2) st_nt(st + m.dsize[C] + flat, psi2);
       const T t = vsub(vadd(vsub(vdiv(vdiv(D1, d1), p1[2]), vdiv(vdiv(D2, d2), p2[2])), psi1), psi2);
       if constexpr (E) return vadd(f, vscl(rec.c0, t));
       else return vsub(f, vscl(rec.c0, t));
@@ -269,7 +269,7 @@ __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typ
     T dn;
     if constexpr (E) dn = vadd(vscl(p1[0], dstore), vscl(p1[1], cu));
     else dn = vsub(vscl(p1[0], dstore), vscl(p1[1], cu));
-    gst(st + flat, dn);
+    st_nt(st + flat, dn);
     return vadd(vscl(p2[2], f), vdiv(vscl(p2[3], vsub(vscl(p0[4], dn), vscl(p0[5], dstore))), rec.eps_mu));
   }
 }
