@@ -911,21 +911,21 @@ __device__ __forceinline__ void ade_drude_cell(const AdeArgs<T>& m, int q) {
   const R* cc = a + 3 * P;
   T* st = static_cast<T*>(m.state);
   T* F = m.fa.f[C] + didx(m.fa, C, i, j, k);
-  const T e_now = *F;
+  const T e_now = ld_nt(F);
   T sum = vzero<T>();
   for (int p = 0; p < P; ++p) {
-    T q_old = st[p * m.n + q];
-    T q_now = st[(P + p) * m.n + q];
-    T q_new = vadd(vadd(vscl(a[3 * p], q_old), vscl(a[3 * p + 1], q_now)), vscl(a[3 * p + 2], e_now));
-    st[p * m.n + q] = q_now;
-    st[(P + p) * m.n + q] = q_new;
+    const T q_old = ld_nt(st + p * m.n + q);
+    const T q_now = ld_nt(st + (P + p) * m.n + q);
+    const T q_new = vadd(vadd(vscl(a[3 * p], q_old), vscl(a[3 * p + 1], q_now)), vscl(a[3 * p + 2], e_now));
+    st_nt(st + p * m.n + q, q_now);
+    st_nt(st + (P + p) * m.n + q, q_new);
+    sum = vadd(sum, vsub(q_new, q_now));
   }
-  for (int p = 0; p < P; ++p) sum = vadd(sum, vsub(st[(P + p) * m.n + q], st[p * m.n + q]));
   T D1, D2;
   curl_terms<C>(m.fa, i, j, k, D1, D2);
   const R d1 = m.d[Ax<C>::a1], d2 = m.d[Ax<C>::a2];
   T cu = vsub(vdiv(D1, d1), vdiv(D2, d2));
-  *F = vadd(vadd(vscl(cc[0], cu), vscl(cc[1], sum)), vscl(cc[2], e_now));
+  st_nt(F, vadd(vadd(vscl(cc[0], cu), vscl(cc[1], sum)), vscl(cc[2], e_now)));
 }
 
 template <int C, class T>
@@ -967,19 +967,19 @@ __device__ __forceinline__ void ade_dcp_cell(const AdeArgs<T>& m, int q) {
   for (int p = 0; p < D; ++p) {
     T qo = st[(1 + p) * n + q], qn = st[(1 + D + p) * n + q];
     T qq = vadd(vadd(vscl(a[3 * p], qo), vscl(a[3 * p + 1], qn)), vscl(a[3 * p + 2], mix));
-    st[(1 + p) * n + q] = qn;
-    st[(1 + D + p) * n + q] = qq;
+    st_nt(st + (1 + p) * n + q, qn);
+    st_nt(st + (1 + D + p) * n + q, qq);
   }
   for (int p = 0; p < NC; ++p) {
     T po = st[(1 + 2 * D + p) * n + q], pn = st[(1 + 2 * D + NC + p) * n + q];
     T pp = vadd(vadd(vadd(vadd(vscl(b[5 * p], po), vscl(b[5 * p + 1], pn)), vscl(b[5 * p + 2], e_old)),
                      vscl(b[5 * p + 3], e_now)),
                 vscl(b[5 * p + 4], e_new));
-    st[(1 + 2 * D + p) * n + q] = pn;
-    st[(1 + 2 * D + NC + p) * n + q] = pp;
+    st_nt(st + (1 + 2 * D + p) * n + q, pn);
+    st_nt(st + (1 + 2 * D + NC + p) * n + q, pp);
   }
-  st[q] = e_now;
-  *F = e_new;
+  st_nt(st + q, e_now);
+  st_nt(F, e_new);
 }
 
 template <int C, class T>
