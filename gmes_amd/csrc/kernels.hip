@@ -1172,12 +1172,18 @@ __global__ void __launch_bounds__(256) k_dft_batch(const DftJob<T>* jobs, int nj
   const double vr = (double)re_of(v), vi = (double)im_of(v);
   // the job table's pointers are global memory: address them as such
   const GMES_GLOBAL double* w = (const GMES_GLOBAL double*)J.w + row * J.nfreq * 4 + (J.electric ? 0 : 2);
-  GMES_GLOBAL double* acc = (GMES_GLOBAL double*)J.acc + 2 * q;
+  // each (re, im) accumulator pair is one 16-byte word, read and written
+  // once per sampled step: non-temporal
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  GMES_GLOBAL d2* acc = (GMES_GLOBAL d2*)J.acc + q;
+#pragma unroll 4
   for (int f = 0; f < J.nfreq; ++f) {
     const double wr = w[4 * f], wi = w[4 * f + 1];
-    GMES_GLOBAL double* a = acc + 2 * f * J.ncells;
-    a[0] = a[0] + (wr * vr - wi * vi);
-    a[1] = a[1] + (wr * vi + wi * vr);
+    GMES_GLOBAL d2* a = acc + f * J.ncells;
+    d2 x = __builtin_nontemporal_load(a);
+    x.x = x.x + (wr * vr - wi * vi);
+    x.y = x.y + (wr * vi + wi * vr);
+    __builtin_nontemporal_store(x, a);
   }
 }
 
