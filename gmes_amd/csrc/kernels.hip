@@ -738,8 +738,10 @@ __device__ __forceinline__ void update_h(const MainArgs<T>& m, const DevRec<type
 // SPLIT: separate update code for records in LDS and in global memory
 // (faster for the single-sim kernel); otherwise one copy reads them
 // through a generic pointer (faster, as measured, for the batched kernel).
-// UP: see prefetch_psi.
-template <bool E, class T, int N, bool SPLIT, bool UP>
+// LDSO: the launch guarantees that every block stages its records in LDS
+// (RecStage::use_lds), so one copy reads them from LDS only (batched
+// kernel, 7% faster than the generic pointer).  UP: see prefetch_psi.
+template <bool E, class T, int N, bool SPLIT, bool UP, bool LDSO = false>
 __device__ __forceinline__ void main3d_tile(const MainArgs<T>& m, DevRec<typename real_of<T>::type>* srec, int k,
                                             int j, int i0, int tid, int nthreads) {
   using R = typename real_of<T>::type;
@@ -778,7 +780,10 @@ __device__ __forceinline__ void main3d_tile(const MainArgs<T>& m, DevRec<typenam
     }
     stage.commit(srec, m.nrec);
     if (!act) return;
-    if constexpr (!SPLIT) {
+    if constexpr (LDSO) {
+      if constexpr (E) update_e<T, N>(m, srec, q, i0, j, k, B, P, W, f[0], f[1], f[2], sx, sy, sz, u0, u1);
+      else update_h<T, N>(m, srec, q, i0, j, k, B, P, W, f[3], f[4], f[5], sx, sy, sz, u0, u1);
+    } else if constexpr (!SPLIT) {
       const DevRec<R>* recs = stage.use_lds ? srec : grec;
       if constexpr (E) update_e<T, N>(m, recs, q, i0, j, k, B, P, W, f[0], f[1], f[2], sx, sy, sz, u0, u1);
       else update_h<T, N>(m, recs, q, i0, j, k, B, P, W, f[3], f[4], f[5], sx, sy, sz, u0, u1);
@@ -792,7 +797,7 @@ __device__ __forceinline__ void main3d_tile(const MainArgs<T>& m, DevRec<typenam
   } else {
     stage.commit(srec, m.nrec);
     if (!act) return;
-    const DevRec<R>* recs = stage.use_lds ? srec : grec;  // remainder planes only
+    const DevRec<R>* recs = LDSO ? srec : (stage.use_lds ? srec : grec);  // remainder planes only
     T u0, u1;
     if constexpr (E) {
       u0 = gld(f[5] + B[5] + i0 * P[5] + (j + 1) * W[5] + k);
@@ -836,7 +841,7 @@ k_main3d(MainArgs<T> m) {
 // (uniform across the block) and runs the same tile body as the single-sim
 // kernel, with the member's own tile shape.
 
-template <bool E, class T>
+template <bool E, class T, bool LDSO>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MainWaves<T, 1>::v)))
 k_main3d_batch(const MainJob<T>* jobs, int njobs) {
   __shared__ DevRec<typename real_of<T>::type> srec[kRecLds];
@@ -845,7 +850,7 @@ k_main3d_batch(const MainJob<T>* jobs, int njobs) {
   const int bxi = local % J.gx, byi = (local / J.gx) % J.gy, bzi = local / (J.gx * J.gy);
   const int tid = threadIdx.x;
   const int tx = tid & (J.bx - 1), ty = tid / J.bx;
-  main3d_tile<E, T, 1, false, true>(J.m, srec, bxi * J.bx + tx, byi * (256 / J.bx) + ty, bzi, tid, 256);
+  main3d_tile<E, T, 1, false, true, LDSO>(J.m, srec, bxi * J.bx + tx, byi * (256 / J.bx) + ty, bzi, tid, 256);
 }
 
 template <bool E, class T>
@@ -1375,9 +1380,12 @@ void launch_main_batch(bool electric, int kind, const MainJob<T>* jobs, int njob
                        hipStream_t st) {
   if (njobs == 0 || nblocks == 0) return;
   const dim3 grid((unsigned)nblocks), block(256);
-  if (kind == kBatch3d) {
-    if (electric) hipLaunchKernelGGL((k_main3d_batch<true, T>), grid, block, 0, st, jobs, njobs);
-    else hipLaunchKernelGGL((k_main3d_batch<false, T>), grid, block, 0, st, jobs, njobs);
+  if (kind == kBatch3dLds) {
+    if (electric) hipLaunchKernelGGL((k_main3d_batch<true, T, true>), grid, block, 0, st, jobs, njobs);
+    else hipLaunchKernelGGL((k_main3d_batch<false, T, true>), grid, block, 0, st, jobs, njobs);
+  } else if (kind == kBatch3d) {
+    if (electric) hipLaunchKernelGGL((k_main3d_batch<true, T, false>), grid, block, 0, st, jobs, njobs);
+    else hipLaunchKernelGGL((k_main3d_batch<false, T, false>), grid, block, 0, st, jobs, njobs);
   } else if (kind == kBatch2z) {
     if (electric) hipLaunchKernelGGL((k_main2z_batch<true, T>), grid, block, 0, st, jobs, njobs);
     else hipLaunchKernelGGL((k_main2z_batch<false, T>), grid, block, 0, st, jobs, njobs);

@@ -89,6 +89,23 @@ def test_complex_batch_members_match_alone(engine_lib, precision):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_with_large_record_table(engine_lib, precision):
+    """A 3-D member with more material records than the LDS staging holds
+    (here 140 spheres of distinct permittivity) makes the batch read the
+    records through the generic path; results still equal the runs alone."""
+    def members():
+        return [spheres3d(engine_lib, precision=precision, n=32, nsph=140, seed=11),
+                spheres3d(engine_lib, precision=precision, n=24)]
+    assert len(members()[0]._tiles.records) > 128
+    alone = run_alone(members(), CHUNKS)
+    batched, _ = run_batched(members(), CHUNKS)
+    for i, (a, b) in enumerate(zip(alone, batched)):
+        for c in a:
+            np.testing.assert_array_equal(a[c], b[c], err_msg="member %d %s" % (i, c.__name__))
+
+
+@pytest.mark.gpu
 def test_batch_matches_oracle(engine_lib, oracle_lib):
     batched, _ = run_batched(real_members(engine_lib, "fp64"), CHUNKS)
     ref = run_alone(real_members(oracle_lib, "fp64"), CHUNKS)
