@@ -327,7 +327,7 @@ def main():
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBPS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBPS, 4),
                     traffic=measured_traffic(n, "E" if fam == 0 else "H"),
-                    kernel="k_main3d<%s, float, 1, false>" % ("true" if fam == 0 else "false"),
+                    kernel="k_main3d<%s, float, 1, false, true>" % ("true" if fam == 0 else "false"),
                     bytes_per_launch=int(nbytes), avg_launch_ms=round(avg_ms, 5),
                     other_kernel_gbps=round(other_gbps, 1))
     fp32_steps_done = args.warmup + args.steps + prof_steps

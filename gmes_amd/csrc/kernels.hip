@@ -826,11 +826,14 @@ __device__ __forceinline__ void main3d_tile(const MainArgs<T>& m, DevRec<typenam
 template <class T, int N> struct MainWaves { static constexpr int v = 1; };
 template <> struct MainWaves<float, 1> { static constexpr int v = 8; };
 
-template <bool E, class T, int N, bool UP>
+// LDSO: the launch guarantees the record table fits the LDS staging (see
+// main3d_tile); otherwise the update code is split into an LDS and a
+// global copy.
+template <bool E, class T, int N, bool UP, bool LDSO>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MainWaves<T, N>::v)))
 k_main3d(MainArgs<T> m) {
   __shared__ DevRec<typename real_of<T>::type> srec[kRecLds];
-  main3d_tile<E, T, N, true, UP>(m, srec, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
+  main3d_tile<E, T, N, !LDSO, UP, LDSO>(m, srec, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
                        blockIdx.z * N, threadIdx.y * blockDim.x + threadIdx.x, blockDim.x * blockDim.y);
 }
 
@@ -1346,16 +1349,24 @@ void launch_main3d(bool electric, const MainArgs<T>& m, int bx, int by, int ichu
   dim3 grid((m.n[2] + bx - 1) / bx, (m.n[1] + by - 1) / by, (m.n[0] + n - 1) / n);
   // the UPML-prefetching variant exists for real FP32 only (the only type
   // whose kernels prefetch state words)
-#define GMES_L3(NN, UP)                                                                   \
-  if (n == NN) {                                                                          \
-    if (electric) hipLaunchKernelGGL((k_main3d<true, T, NN, UP>), grid, block, 0, st, m); \
-    else hipLaunchKernelGGL((k_main3d<false, T, NN, UP>), grid, block, 0, st, m);         \
-    return;                                                                               \
+  // one plane per block (the default) with a record table that fits the
+  // block's LDS staging (RecStage::use_lds): the LDS-only variant
+  constexpr int chunks = (int)(sizeof(DevRec<typename real_of<T>::type>) / 16);
+  const bool lds = m.nrec <= kRecLds && m.nrec * chunks <= bx * by;
+#define GMES_L3(NN, UP, LDSO)                                                                   \
+  if (n == NN) {                                                                                \
+    if (electric) hipLaunchKernelGGL((k_main3d<true, T, NN, UP, LDSO>), grid, block, 0, st, m); \
+    else hipLaunchKernelGGL((k_main3d<false, T, NN, UP, LDSO>), grid, block, 0, st, m);         \
+    return;                                                                                     \
   }
   if constexpr (std::is_same<T, float>::value) {
-    if (m.upml_mask) { GMES_L3(1, true) GMES_L3(2, true) GMES_L3(4, true) GMES_L3(8, true) }
+    if (m.upml_mask) {
+      if (lds) GMES_L3(1, true, true)
+      GMES_L3(1, true, false) GMES_L3(2, true, false) GMES_L3(4, true, false) GMES_L3(8, true, false)
+    }
   }
-  GMES_L3(1, false) GMES_L3(2, false) GMES_L3(4, false) GMES_L3(8, false)
+  if (lds) GMES_L3(1, false, true)
+  GMES_L3(1, false, false) GMES_L3(2, false, false) GMES_L3(4, false, false) GMES_L3(8, false, false)
 #undef GMES_L3
 }
 
