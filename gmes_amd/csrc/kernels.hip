@@ -819,18 +819,23 @@ __device__ __forceinline__ void main3d_tile(const MainArgs<T>& m, DevRec<typenam
   }
 }
 
-// Waves per SIMD the register allocator must leave room for.  Real FP32
-// with one plane per thread fits 64 VGPRs (8 waves) by moving a few scalar
-// registers to VGPR lanes: 0.8% faster on 256^3 than 7 waves.  FP64 forced
-// from 5 to 6 waves spills to scratch and is 13% slower, so it stays free.
-template <class T, int N> struct MainWaves { static constexpr int v = 1; };
-template <> struct MainWaves<float, 1> { static constexpr int v = 8; };
+// Waves per SIMD the register allocator must leave room for (L: the
+// LDS-only record variant, which needs far fewer registers).  Real FP32
+// with one plane per thread fits 64 VGPRs (8 waves); with split record
+// copies it gets there by moving a few scalar registers to VGPR lanes,
+// 0.8% faster on 256^3 than 7 waves.  FP64 with split copies forced from 5
+// to 6 waves spills to scratch and is 13% slower, so it stays free; the
+// LDS-only FP64 kernel fits 7 waves without spills (256^3 0.784 to 0.730
+// ms per step), while 8 waves spills and is 4% slower than 7.
+template <class T, int N, bool L = false> struct MainWaves { static constexpr int v = 1; };
+template <bool L> struct MainWaves<float, 1, L> { static constexpr int v = 8; };
+template <> struct MainWaves<double, 1, true> { static constexpr int v = 7; };
 
 // LDSO: the launch guarantees the record table fits the LDS staging (see
 // main3d_tile); otherwise the update code is split into an LDS and a
 // global copy.
 template <bool E, class T, int N, bool UP, bool LDSO>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MainWaves<T, N>::v)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MainWaves<T, N, LDSO>::v)))
 k_main3d(MainArgs<T> m) {
   __shared__ DevRec<typename real_of<T>::type> srec[kRecLds];
   main3d_tile<E, T, N, !LDSO, UP, LDSO>(m, srec, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
