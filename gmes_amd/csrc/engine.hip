@@ -1717,6 +1717,7 @@ struct gmes_batch {
   struct JobSet {
     Launch main3d[2], main2z[2], mainnd[2], src[2], ghost[2];  // [0] E, [1] H
     bool lds3d[2] = {false, false};  // every 3-D job stages its records in LDS
+    bool lds2z[2] = {false, false};  // every collapsed-z job does
     Launch ade[GMES_NCLS];                                      // by dispersive class
     Launch dft;
   };
@@ -1743,7 +1744,9 @@ void build_jobs(gmes_batch* b, const std::vector<gmes_sim*>& sims, gmes_batch::J
     const bool electric = e == 0;
     std::vector<MainJob<T>> j3, j2, jn;
     int64_t nb3 = 0, nb2 = 0, nbn = 0;
-    bool lds3d = true;
+    bool lds3d = true, lds2z = true;
+    // the block-wide record staging of the main tiles (RecStage::use_lds)
+    constexpr int chunks = (int)(sizeof(DevRec<typename real_of<T>::type>) / 16);
     std::vector<SrcJob<T>> js;
     int64_t nbs = 0;
     std::vector<GhostJob<T>> jg;
@@ -1757,8 +1760,6 @@ void build_jobs(gmes_batch* b, const std::vector<gmes_sim*>& sims, gmes_batch::J
         J.bx = bx;
         const int tby = 256 / bx;
         if (is3d(s)) {
-          // the block-wide record staging of the 3-D tile (RecStage::use_lds)
-          constexpr int chunks = (int)(sizeof(DevRec<typename real_of<T>::type>) / 16);
           if (J.m.nrec > kRecLds || J.m.nrec * chunks > 256) lds3d = false;
           J.gx = (s->g.nz + bx - 1) / bx;
           J.gy = (s->g.ny + tby - 1) / tby;
@@ -1766,6 +1767,7 @@ void build_jobs(gmes_batch* b, const std::vector<gmes_sim*>& sims, gmes_batch::J
           nb3 += (int64_t)J.gx * J.gy * s->g.nx;
           j3.push_back(J);
         } else if (is_2z(s)) {
+          if (J.m.nrec > kRecLds || J.m.nrec * chunks > 256) lds2z = false;
           J.bx = 256;
           J.gx = (ext[1] + 255) / 256;
           J.gy = (ext[0] + k2zRows - 1) / k2zRows;
@@ -1797,6 +1799,7 @@ void build_jobs(gmes_batch* b, const std::vector<gmes_sim*>& sims, gmes_batch::J
     }
     out.main3d[e] = {append_table(b->h_jobs, j3.data(), j3.size() * sizeof(MainJob<T>)), (int)j3.size(), nb3};
     out.lds3d[e] = lds3d;
+    out.lds2z[e] = lds2z;
     out.main2z[e] = {append_table(b->h_jobs, j2.data(), j2.size() * sizeof(MainJob<T>)), (int)j2.size(), nb2};
     out.mainnd[e] = {append_table(b->h_jobs, jn.data(), jn.size() * sizeof(MainJob<T>)), (int)jn.size(), nbn};
     out.src[e] = {append_table(b->h_jobs, js.data(), js.size() * sizeof(SrcJob<T>)), (int)js.size(), nbs};
@@ -1844,7 +1847,8 @@ void enqueue_set_step_t(gmes_batch* b, const gmes_batch::JobSet& js, const std::
     launch_main_batch<T>(electric, js.lds3d[e] ? kBatch3dLds : kBatch3d, job_ptr<MainJob<T>>(b, js.main3d[e].off),
                          js.main3d[e].njobs,
                          js.main3d[e].nblocks, st);
-    launch_main_batch<T>(electric, kBatch2z, job_ptr<MainJob<T>>(b, js.main2z[e].off), js.main2z[e].njobs,
+    launch_main_batch<T>(electric, js.lds2z[e] ? kBatch2zLds : kBatch2z, job_ptr<MainJob<T>>(b, js.main2z[e].off),
+                         js.main2z[e].njobs,
                          js.main2z[e].nblocks, st);
     launch_main_batch<T>(electric, kBatchGeneric, job_ptr<MainJob<T>>(b, js.mainnd[e].off), js.mainnd[e].njobs,
                          js.mainnd[e].nblocks, st);

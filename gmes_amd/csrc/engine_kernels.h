@@ -191,9 +191,9 @@ void launch_main(bool electric, const MainArgs<T>& m, int ext[3], int bx, int by
 // and 4 at 4096^2; 1.4x the generic kernel in FP32)
 constexpr int k2zRows = 1;
 // kinds of batched main launch
-// kBatch3dLds: 3-D jobs whose record tables all fit the LDS staging
-// (RecStage::use_lds for 256-thread blocks)
-constexpr int kBatchGeneric = 0, kBatch3d = 1, kBatch2z = 2, kBatch3dLds = 3;
+// kBatch3dLds / kBatch2zLds: 3-D / collapsed-z jobs whose record tables
+// all fit the LDS staging (RecStage::use_lds for 256-thread blocks)
+constexpr int kBatchGeneric = 0, kBatch3d = 1, kBatch2z = 2, kBatch3dLds = 3, kBatch2zLds = 4;
 template <class T>
 void launch_main_batch(bool electric, int kind, const MainJob<T>* jobs, int njobs, int64_t nblocks,
                        hipStream_t st);

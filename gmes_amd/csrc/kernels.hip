@@ -499,7 +499,9 @@ __device__ __forceinline__ void h_zghost(const MainArgs<T>& m, int i0, int j) {
   }
 }
 
-template <bool E, class T, int N>
+// LDSO: the launch guarantees every block stages its records in LDS, so
+// only the LDS copy of the update code is built (see main3d_tile).
+template <bool E, class T, int N, bool LDSO = false>
 __device__ __forceinline__ void main2z_tile(const MainArgs<T>& m, DevRec<typename real_of<T>::type>* srec, int i0,
                                             int j, int tid, int nthreads) {
   using R = typename real_of<T>::type;
@@ -511,7 +513,7 @@ __device__ __forceinline__ void main2z_tile(const MainArgs<T>& m, DevRec<typenam
   load2z<C0 + 1, T, N>(m, q, i0, j);
   load2z<C0 + 2, T, N>(m, q, i0, j);
   stage.commit(srec, m.nrec);
-  if (stage.use_lds) {
+  if (LDSO || stage.use_lds) {
     update2z<C0 + 0, T, N>(m, srec, q, i0, j);
     update2z<C0 + 1, T, N>(m, srec, q, i0, j);
     update2z<C0 + 2, T, N>(m, srec, q, i0, j);
@@ -878,13 +880,13 @@ __global__ void __launch_bounds__(256) k_main_batch(const MainJob<T>* jobs, int 
 
 // Collapsed-z 2-D members: a job's blocks are gx blocks of 256 j by gy
 // groups of k2zRows i rows.
-template <bool E, class T>
+template <bool E, class T, bool LDSO>
 __global__ void __launch_bounds__(256) k_main2z_batch(const MainJob<T>* jobs, int njobs) {
   __shared__ DevRec<typename real_of<T>::type> srec[kRecLds];
   const MainJob<T>& J = jobs[find_job(jobs, njobs, blockIdx.x)];
   const int local = (int)(blockIdx.x - J.block0);
   const int bxi = local % J.gx, byi = local / J.gx;
-  main2z_tile<E, T, k2zRows>(J.m, srec, byi * k2zRows, bxi * 256 + (int)threadIdx.x, threadIdx.x, 256);
+  main2z_tile<E, T, k2zRows, LDSO>(J.m, srec, byi * k2zRows, bxi * 256 + (int)threadIdx.x, threadIdx.x, 256);
 }
 
 // Batched launches: block -> job by binary search on the jobs' first
@@ -1402,9 +1404,12 @@ void launch_main_batch(bool electric, int kind, const MainJob<T>* jobs, int njob
   } else if (kind == kBatch3d) {
     if (electric) hipLaunchKernelGGL((k_main3d_batch<true, T, false>), grid, block, 0, st, jobs, njobs);
     else hipLaunchKernelGGL((k_main3d_batch<false, T, false>), grid, block, 0, st, jobs, njobs);
+  } else if (kind == kBatch2zLds) {
+    if (electric) hipLaunchKernelGGL((k_main2z_batch<true, T, true>), grid, block, 0, st, jobs, njobs);
+    else hipLaunchKernelGGL((k_main2z_batch<false, T, true>), grid, block, 0, st, jobs, njobs);
   } else if (kind == kBatch2z) {
-    if (electric) hipLaunchKernelGGL((k_main2z_batch<true, T>), grid, block, 0, st, jobs, njobs);
-    else hipLaunchKernelGGL((k_main2z_batch<false, T>), grid, block, 0, st, jobs, njobs);
+    if (electric) hipLaunchKernelGGL((k_main2z_batch<true, T, false>), grid, block, 0, st, jobs, njobs);
+    else hipLaunchKernelGGL((k_main2z_batch<false, T, false>), grid, block, 0, st, jobs, njobs);
   } else {
     if (electric) hipLaunchKernelGGL((k_main_batch<true, T>), grid, block, 0, st, jobs, njobs);
     else hipLaunchKernelGGL((k_main_batch<false, T>), grid, block, 0, st, jobs, njobs);

@@ -88,16 +88,35 @@ def test_complex_batch_members_match_alone(engine_lib, precision):
             np.testing.assert_array_equal(a[c], b[c], err_msg="member %d %s" % (i, c.__name__))
 
 
+def tmz_many(lib, precision):
+    """TMz with 140 cylinders of distinct permittivity (141+ records)."""
+    rng = np.random.default_rng(12)
+    space = g.Cartesian(size=(12, 12, 0), resolution=6)
+    geometry = [g.DefaultMedium(material=g.Dielectric())]
+    for _ in range(140):
+        c = rng.uniform(-4.5, 4.5, 2)
+        geometry.append(g.Cylinder(g.Dielectric(float(rng.uniform(1.5, 9))), center=(c[0], c[1], 0),
+                                   radius=0.3, axis=(0, 0, 1)))
+    geometry.append(g.Shell(g.Cpml(), thickness=1))
+    src = [g.PointSource(g.Bandpass(freq=0.6, fwidth=0.2), center=(0.1, -0.2, 0), component=g.Ez)]
+    sim = g.TMzFDTD(space, geometry, src, verbose=False, precision=precision, _engine=lib)
+    sim.init()
+    return sim
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_batch_with_large_record_table(engine_lib, precision):
-    """A 3-D member with more material records than the LDS staging holds
-    (here 140 spheres of distinct permittivity) makes the batch read the
-    records through the generic path; results still equal the runs alone."""
+    """A 3-D and a TMz member with more material records than the LDS
+    staging holds (140 spheres / cylinders of distinct permittivity) make
+    the batch read the records through the generic path; results still
+    equal the runs alone."""
     def members():
         return [spheres3d(engine_lib, precision=precision, n=32, nsph=140, seed=11),
-                spheres3d(engine_lib, precision=precision, n=24)]
-    assert len(members()[0]._tiles.records) > 128
+                spheres3d(engine_lib, precision=precision, n=24),
+                tmz_many(engine_lib, precision), kat_tmz(engine_lib, precision=precision)]
+    m = members()
+    assert len(m[0]._tiles.records) > 128 and len(m[2]._tiles.records) > 128
     alone = run_alone(members(), CHUNKS)
     batched, _ = run_batched(members(), CHUNKS)
     for i, (a, b) in enumerate(zip(alone, batched)):
