@@ -825,7 +825,8 @@ __device__ __forceinline__ void main3d_tile(const MainArgs<T>& m, DevRec<typenam
 // LDS-only record variant, which needs far fewer registers).  Real FP32
 // with one plane per thread fits 64 VGPRs (8 waves); with split record
 // copies it gets there by moving a few scalar registers to VGPR lanes,
-// 0.8% faster on 256^3 than 7 waves.  FP64 with split copies forced from 5
+// 0.8% faster on 256^3 than 7 waves.  The LDS-only FP32 kernels reach 8
+// waves without the hint too, but ran 1% slower without it.  FP64 with split copies forced from 5
 // to 6 waves spills to scratch and is 13% slower, so it stays free; the
 // LDS-only FP64 kernel fits 7 waves without spills (256^3 0.784 to 0.730
 // ms per step), while 8 waves spills and is 4% slower than 7.
