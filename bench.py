@@ -90,6 +90,28 @@ def time_steps(sim, lib, steps):
     return time.perf_counter() - t0
 
 
+def fp32_vs_fp64(sim32, sim64, lib, horizon):
+    """Relative L2 error of the FP32 fields against the FP64 fields of the
+    same workload, both advanced (untimed) to the same step ``horizon``."""
+    from gmes_amd import constant as const
+    for s in (sim32, sim64):
+        done = lib.get_step(s._sim)
+        s._advance(horizon - done)  # raises if a sim is already past the horizon
+        lib.sync(s._sim)
+    steps = {lib.get_step(s._sim) for s in (sim32, sim64)}
+    if steps != {horizon}:
+        raise RuntimeError("FP32 and FP64 sims are at steps %s, not %d" % (sorted(steps), horizon))
+    num = den = 0.0
+    for c in const.COMPONENTS:
+        a = np.asarray(sim32.field[c], np.float64)
+        b = np.asarray(sim64.field[c], np.float64)
+        num += float(np.sum((a - b) ** 2))
+        den += float(np.sum(b ** 2))
+    err = dict(steps=horizon, rel_l2_fp32_vs_fp64=(num / den) ** 0.5 if den else 0.0, bound=1e-5)
+    log("fp32 vs fp64 after %d steps: %.3g" % (horizon, err["rel_l2_fp32_vs_fp64"]))
+    return err
+
+
 def cpu_baseline(n, steps, warm):
     """The reference's own updaters (oracle/_ref/libgmes_ref.so, compiled
     from the reference's src/pw_*.hh) in the reference's step order, FP64,
@@ -266,6 +288,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256, help="cells per axis")
     ap.add_argument("--fp64-steps", type=int, default=500)
+    ap.add_argument("--error-steps", type=int, default=2000,
+                    help="step at which the FP32 fields are compared with the FP64 fields")
     ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
@@ -331,9 +355,6 @@ def main():
                     bytes_per_launch=int(nbytes), avg_launch_ms=round(avg_ms, 5),
                     other_kernel_gbps=round(other_gbps, 1))
     fp32_steps_done = args.warmup + args.steps + prof_steps
-    from gmes_amd import constant as const
-    fp32_fields = {c: np.array(sim.field[c]) for c in const.COMPONENTS}
-    sim.close()
 
     # ---- FP64 (the reference's precision) --------------------------------
     fp64 = None
@@ -342,25 +363,16 @@ def main():
         sim64 = workloads.spheres3d(n, precision="fp64")
         lib.set_graph(sim64._sim, 0 if args.no_graph else 1)
         lib.set_ghost_fusion(sim64._sim, 0 if args.no_ghost_fusion else 1)
-        sim64._advance(min(args.warmup, 10))
+        warm64 = min(args.warmup, 10)
+        sim64._advance(warm64)
         s64 = time_steps(sim64, lib, args.fp64_steps)
         fp64 = dict(value=round(cells * args.fp64_steps / s64 / 1e6, 1), unit="Mcell-updates/s",
                     steps=args.fp64_steps, ms_per_step=round(s64 / args.fp64_steps * 1e3, 4))
         log("fp64: %d steps in %.3f s -> %.1f Mcell/s" % (args.fp64_steps, s64, fp64["value"]))
-        # FP32 error against FP64 at the same step (the FP32 run above)
-        done64 = min(args.warmup, 10) + args.fp64_steps
-        sim64._advance(fp32_steps_done - done64)
-        lib.sync(sim64._sim)
-        from gmes_amd import constant as const
-        num = den = 0.0
-        for c in const.COMPONENTS:
-            a = np.asarray(fp32_fields[c], np.float64)
-            b = np.asarray(sim64.field[c], np.float64)
-            num += float(np.sum((a - b) ** 2))
-            den += float(np.sum(b ** 2))
-        fp32_err = dict(steps=fp32_steps_done, rel_l2_fp32_vs_fp64=(num / den) ** 0.5 if den else 0.0)
-        log("fp32 vs fp64 after %d steps: %.3g" % (fp32_steps_done, fp32_err["rel_l2_fp32_vs_fp64"]))
+        fp32_err = fp32_vs_fp64(sim, sim64, lib, max(args.error_steps, fp32_steps_done,
+                                                     warm64 + args.fp64_steps))
         sim64.close()
+    sim.close()
 
     # ---- batched instances -----------------------------------------------
     batched = None

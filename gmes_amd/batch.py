@@ -45,6 +45,19 @@ class Batch(object):
     def sync(self):
         self._lib.batch_sync(self._b)
 
+    def member_errors(self):
+        """Wait for the batch and return {member index: message} for the
+        members whose kernels reported a deferred error (the DM2 corrector's
+        non-convergence or invalid error, src/pw_dm2.hh:398-402).  The flag
+        is per member, so the other members' results stand."""
+        out = {}
+        for i, s in enumerate(self.sims):
+            try:
+                self._lib.sync(s._sim)
+            except RuntimeError as e:
+                out[i] = str(e)
+        return out
+
     def close(self):
         """Dissolve the batch (the members stay usable on their own)."""
         if self._b is not None:

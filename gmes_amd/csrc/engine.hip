@@ -1003,6 +1003,18 @@ int gmes_sim_create(const gmes_grid_desc* desc, gmes_sim** out) {
   if (!desc || !out) return set_error(GMES_EINVAL, "null argument");
   if (desc->dtype < GMES_F32 || desc->dtype > GMES_C128) return set_error(GMES_ETYPE, "bad dtype");
   if (desc->nx < 1 || desc->ny < 1 || desc->nz < 1) return set_error(GMES_EINVAL, "grid sizes must be positive");
+  // Kernels index a component with 32-bit offsets (and ADE cell lists hold
+  // 32-bit cells): refuse any component of 2^31 device elements or more,
+  // before touching the device.
+  for (int c = 0; c < 6; ++c) {
+    if (!((desc->active >> c) & 1u)) continue;
+    const int64_t ext[3] = {(int64_t)desc->nx + (c % 3 != 0), (int64_t)desc->ny + (c % 3 != 1),
+                            (int64_t)desc->nz + (c % 3 != 2)};
+    const int64_t row = ext[2] >= 32 ? (ext[2] + 31) / 32 * 32 : ext[2];
+    const double elems = 2.0 * kPad + (double)ext[0] * (double)ext[1] * (double)row;
+    if (elems >= 2147483648.0)
+      return set_error(GMES_EINVAL, "component %d would hold %.0f device elements; the limit is 2^31 - 1", c, elems);
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(GMES_EDEVICE, "no GPU visible");
   if (desc->device < 0 || desc->device >= ndev) return set_error(GMES_EINVAL, "bad device ordinal");
@@ -1533,6 +1545,15 @@ int gmes_sim_read_field(gmes_sim* s, int32_t c, void* host, int64_t bytes) {
   return GMES_OK;
 }
 
+// ---- snapshot / restore
+// The blob of a sim: a header (magic, step), the CPML/UPML and ADE state,
+// the DFT accumulators, and for every attached auxiliary grid its six
+// fields followed by its own blob.  The sim's own six fields are not in
+// it: the caller keeps them (gmes_sim_read_field / gmes_sim_write_field).
+// Restore sets the step clock back to the snapshot's and drops unread
+// probe values, so the next step records and injects at the restored time.
+constexpr int64_t kSnapMagic = 0x31504e5353454d47LL;  // "GMESSNP1"
+
 static void state_blocks(gmes_sim* s, std::vector<std::pair<void*, size_t>>& out) {
   for (int c = 0; c < 6; ++c) {
     if (s->pml_state[c]) {
@@ -1542,47 +1563,95 @@ static void state_blocks(gmes_sim* s, std::vector<std::pair<void*, size_t>>& out
     for (int k = GMES_CLS_DRUDE; k < GMES_NCLS; ++k)
       if (s->ade[c][k].n) out.push_back({s->ade[c][k].state, (size_t)s->ade[c][k].nstate * s->ade[c][k].n * ade_word(s, k)});
   }
+  for (auto& d : s->dfts) out.push_back({d.acc, (size_t)d.nfreq * d.ncells * 2 * sizeof(double)});
+}
+
+static int64_t blob_bytes(gmes_sim* s) {
+  std::vector<std::pair<void*, size_t>> b;
+  state_blocks(s, b);
+  int64_t t = 2 * sizeof(int64_t);
+  for (auto& x : b) t += (int64_t)x.second;
+  for (int i = 0; i < s->naux; ++i) {
+    if (!s->aux[i]) continue;
+    for (int c = 0; c < 6; ++c) t += s->aux[i]->dsize[c] * s->aux[i]->esize;
+    t += blob_bytes(s->aux[i]);
+  }
+  return t;
+}
+
+// Copies the blob of s to or from host memory at p (the streams of s are
+// idle); returns the bytes done.
+static int blob_copy(gmes_sim* s, char* p, bool save, int64_t& done) {
+  int64_t head[2] = {kSnapMagic, s->step};
+  if (save) {
+    memcpy(p, head, sizeof head);
+  } else {
+    memcpy(head, p, sizeof head);
+    if (head[0] != kSnapMagic || head[1] < 0) return set_error(GMES_EINVAL, "not a snapshot of this engine");
+  }
+  done = sizeof head;
+  std::vector<std::pair<void*, size_t>> b;
+  state_blocks(s, b);
+  for (auto& x : b) {
+    if (save) HIP_TRY(hipMemcpy(p + done, x.first, x.second, hipMemcpyDeviceToHost));
+    else HIP_TRY(hipMemcpy(x.first, p + done, x.second, hipMemcpyHostToDevice));
+    done += (int64_t)x.second;
+  }
+  for (int i = 0; i < s->naux; ++i) {
+    gmes_sim* a = s->aux[i];
+    if (!a) continue;
+    for (int c = 0; c < 6; ++c) {
+      const size_t fb = (size_t)a->dsize[c] * a->esize;
+      if (save) HIP_TRY(hipMemcpy(p + done, a->f[c], fb, hipMemcpyDeviceToHost));
+      else HIP_TRY(hipMemcpy(a->f[c], p + done, fb, hipMemcpyHostToDevice));
+      done += (int64_t)fb;
+    }
+    int64_t sub = 0;
+    int rc = blob_copy(a, p + done, save, sub);
+    if (rc) return rc;
+    done += sub;
+  }
+  if (!save) {
+    s->step = head[1];
+    s->probe_base = s->step;
+    s->probe_n = 0;
+    std::fill(s->probe_read.begin(), s->probe_read.end(), 0);
+    Ctl c;
+    HIP_TRY(hipMemcpy(&c, s->d_clock, sizeof c, hipMemcpyDeviceToHost));
+    c.clock = s->step;
+    c.probe_base = s->probe_base;
+    c.err = 0;
+    HIP_TRY(hipMemcpy(s->d_clock, &c, sizeof c, hipMemcpyHostToDevice));
+  }
+  return GMES_OK;
 }
 
 int gmes_sim_state_bytes(const gmes_sim* s, int64_t* bytes) {
   ApiGuard api;
-  if (!s) return set_error(GMES_EINVAL, "null sim handle");
-  std::vector<std::pair<void*, size_t>> b;
-  state_blocks(const_cast<gmes_sim*>(s), b);
-  int64_t t = 0;
-  for (auto& x : b) t += (int64_t)x.second;
-  *bytes = t;
+  if (!s || !bytes) return set_error(GMES_EINVAL, "null argument");
+  *bytes = blob_bytes(const_cast<gmes_sim*>(s));
   return GMES_OK;
 }
 
 int gmes_sim_snapshot(gmes_sim* s, void* host, int64_t bytes) {
   ApiGuard api;
-  if (!s) return set_error(GMES_EINVAL, "null sim handle");
-  std::vector<std::pair<void*, size_t>> b;
-  state_blocks(s, b);
-  int64_t t = 0;
-  for (auto& x : b) t += (int64_t)x.second;
-  if (t != bytes) return set_error(GMES_EINVAL, "size mismatch");
+  if (!s || !host) return set_error(GMES_EINVAL, "null argument");
+  if (blob_bytes(s) != bytes) return set_error(GMES_EINVAL, "size mismatch");
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(sync_exec(s));
-  char* p = static_cast<char*>(host);
-  for (auto& x : b) { HIP_TRY(hipMemcpy(p, x.first, x.second, hipMemcpyDeviceToHost)); p += x.second; }
-  return GMES_OK;
+  int64_t done = 0;
+  return blob_copy(s, static_cast<char*>(host), true, done);
 }
 
 int gmes_sim_restore(gmes_sim* s, const void* host, int64_t bytes) {
   ApiGuard api;
-  if (!s) return set_error(GMES_EINVAL, "null sim handle");
-  std::vector<std::pair<void*, size_t>> b;
-  state_blocks(s, b);
-  int64_t t = 0;
-  for (auto& x : b) t += (int64_t)x.second;
-  if (t != bytes) return set_error(GMES_EINVAL, "size mismatch");
+  if (!s || !host) return set_error(GMES_EINVAL, "null argument");
+  if (s->parent) return set_error(GMES_EINVAL, "an auxiliary grid is restored with its parent");
+  if (blob_bytes(s) != bytes) return set_error(GMES_EINVAL, "size mismatch");
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(sync_exec(s));
-  const char* p = static_cast<const char*>(host);
-  for (auto& x : b) { HIP_TRY(hipMemcpy(x.first, p, x.second, hipMemcpyHostToDevice)); p += x.second; }
-  return GMES_OK;
+  int64_t done = 0;
+  return blob_copy(s, const_cast<char*>(static_cast<const char*>(host)), false, done);
 }
 
 int gmes_sim_step(gmes_sim* s, int64_t nsteps) {

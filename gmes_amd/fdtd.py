@@ -24,7 +24,7 @@ from numpy import array, inf
 from . import _abi
 from . import constant as const
 from .constant import Electric, Ex, Ey, Ez, Hx, Hy, Hz, Magnetic, X, Y, Z
-from .file_io import Probe
+from .file_io import ProbeFile
 from .geometry import DefaultMedium, GeomBoxTree, in_range
 from .pygeom import GeomBox
 from .tiles import build_tiles
@@ -63,7 +63,10 @@ class _FieldDict(dict):
 
 
 class PwMaterialView(object):
-    """Compatibility view of one reference updater list: idx_size(), name()."""
+    """Compatibility view of one reference updater list: idx_size(), name().
+    ``view_type`` gives each list the type name of the reference's updater
+    (``DielectricEzReal``, ``CpmlHxCmplx``, ...), which scripts read with
+    ``type(updater).__name__`` (benchmarks/field_updates.py:83-96)."""
 
     def __init__(self, name, count):
         self._name = name
@@ -76,9 +79,15 @@ class PwMaterialView(object):
         return self._count
 
 
-_CLASS_NAMES = {_abi.CLS_DUMMY: "Dummy", _abi.CLS_CONST: "Const", _abi.CLS_DIELECTRIC: "Dielectric",
-                _abi.CLS_CPML: "Cpml", _abi.CLS_UPML: "Upml", _abi.CLS_DRUDE: "Drude",
-                _abi.CLS_DCP_ADE: "DcpAde", _abi.CLS_DCP_PLRC: "DcpPlrc", _abi.CLS_DM2: "Dm2"}
+_VIEW_TYPES = {}
+
+
+def view_type(name):
+    """The PwMaterialView subclass called ``name`` (one per name)."""
+    if name not in _VIEW_TYPES:
+        _VIEW_TYPES[name] = type(name, (PwMaterialView,), {"__module__": __name__})
+    return _VIEW_TYPES[name]
+
 
 
 class FDTD(object):
@@ -252,10 +261,11 @@ class FDTD(object):
         for (p, ci, a), vals in tiles.profiles.items():
             lib.set_profile(self._sim, p, ci, a, vals)
         for c in comps:
-            self.pw_material[c] = {
-                _CLASS_NAMES[k] + c.__name__ + ("Cmplx" if self.cmplx else "Real"):
-                    PwMaterialView(_CLASS_NAMES[k] + ("Electric" if c in const.E_COMPONENTS else "Magnetic"), v)
-                for k, v in tiles.class_counts[c].items()}
+            views = {}
+            for prefix, v in tiles.type_counts[c].items():
+                klass = view_type(prefix + c.__name__ + ("Cmplx" if self.cmplx else "Real"))
+                views[klass] = klass(prefix + ("Electric" if c in const.E_COMPONENTS else "Magnetic"), v)
+            self.pw_material[c] = views
         lib.commit(self._sim)
 
         self._aux_fdtds = []
@@ -402,7 +412,10 @@ class FDTD(object):
 
     def _advance(self, nsteps):
         """Run nsteps whole steps on the device and update the bookkeeping."""
-        if nsteps <= 0:
+        nsteps = int(nsteps)
+        if nsteps < 0:
+            raise ValueError("cannot advance by a negative number of steps (%d)" % nsteps)
+        if nsteps == 0:
             return
         if self._sim is None:
             raise RuntimeError("init() must be called before stepping")
@@ -420,14 +433,15 @@ class FDTD(object):
             if aux._sim is not None:
                 aux._after_steps(nsteps)
         if self.e_recorder or self.h_recorder:
+            # drain the device ring buffer: E samples are at n = k + 1/2,
+            # H samples at n = k + 1 after step k
             self._lib.sync(self._sim)
             n_last = self.time_step.n
             for probe in self.e_recorder + self.h_recorder:
-                vals = self._lib.read_probe(self._sim, probe._pid)
+                vals = self._lib.read_probe(self._sim, probe.pid)
                 half = 0.5 if probe in self.e_recorder else 0.0
-                for s, v in enumerate(vals):
-                    n = n_last - len(vals) + s + 1 - half
-                    probe.write_value(n, v if self.cmplx else v.real)
+                steps = [n_last - len(vals) + s + 1 - half for s in range(len(vals))]
+                probe.append(steps, vals if self.cmplx else vals.real)
 
     def add_dft_flux(self, freqs, z, decimation=1):
         """A running-DFT flux monitor on the plane z (whole x-y extent) at
@@ -529,13 +543,11 @@ class FDTD(object):
             if not in_range(idx, self._shape[comp], comp):
                 continue
             filename = (prefix + postfix[comp]) if prefix else postfix[comp]
-            recorder = Probe(idx, None, filename)
             loc = self.space.index_to_space(comp, *idx)
-            recorder.write_header(loc, self.time_step.dt)
+            recorder = ProbeFile(filename, loc, self.time_step.dt, self._arrays[comp].dtype)
             shape = self._shape[comp]
             cell = (idx[0] * shape[1] + idx[1]) * shape[2] + idx[2]
-            recorder._pid = self._lib.add_probe(self._sim, comp.index, cell)
-            recorder._dtype = self._arrays[comp].dtype
+            recorder.pid = self._lib.add_probe(self._sim, comp.index, cell)
             if comp in const.E_COMPONENTS:
                 self.e_recorder.append(recorder)
             else:
@@ -596,8 +608,8 @@ class FDTD(object):
         pnt[idx] = cut
         cut_idx = self.space.space_to_index(comp, *pnt)[idx]
         data = np.take(self._host(comp), cut_idx, axis=idx)
-        from .file_io import snapshot
-        snapshot(data, "t=" + str(self.time_step.t), comp.str())
+        from .file_io import save_png
+        save_png(data, "t=" + str(self.time_step.t), comp.str())
 
     def show_field(self, *args, **kwargs):
         raise NotImplementedError("live plotting is not part of this engine")
@@ -608,20 +620,35 @@ class FDTD(object):
     # ---- state ----------------------------------------------------------
 
     def snapshot(self):
-        """Fields and all auxiliary state (for pre-emption / restart)."""
+        """Fields, time and all auxiliary state, for pre-emption and
+        restart: the engine's blob holds the step clock, CPML/UPML and
+        dispersive state, running-DFT accumulators and the auxiliary TFSF
+        grids (their fields and state).  ``restore`` puts all of it back."""
         self._push_dirty()
+        for aux in self._aux_fdtds:
+            aux._push_dirty()
         self._lib.sync(self._sim)
         return dict(fields={c: self._host(c).copy() for c in const.COMPONENTS},
-                    state=self._lib.snapshot(self._sim), n=self.time_step.n)
+                    state=self._lib.snapshot(self._sim), n=self.time_step.n,
+                    aux_n=[aux.time_step.n for aux in self._aux_fdtds])
 
     def restore(self, snap):
+        """Return to a snapshot taken by ``snapshot`` of this simulation."""
+        if len(snap["aux_n"]) != len(self._aux_fdtds):
+            raise ValueError("snapshot of a different simulation")
+        self._lib.restore(self._sim, snap["state"])
         for c in const.COMPONENTS:
             self._arrays[c][...] = snap["fields"][c]
             self._host_dirty[c] = True
             self._dev_newer[c] = False
-        self._lib.restore(self._sim, snap["state"])
         self.time_step.n = snap["n"]
         self.time_step.t = self.time_step.n * self.time_step.dt
+        for aux, n in zip(self._aux_fdtds, snap["aux_n"]):
+            aux.time_step.n = n
+            aux.time_step.t = n * aux.time_step.dt
+            for c in const.COMPONENTS:
+                aux._host_dirty[c] = False
+                aux._dev_newer[c] = True
 
     def __deepcopy__(self, memo=None):
         """Rebuild and copy the six fields and the time; auxiliary state is
@@ -638,7 +665,18 @@ class FDTD(object):
         for comp in ("ex", "ey", "ez", "hx", "hy", "hz"):
             np.copyto(getattr(newcopy, comp), getattr(self, comp))
         newcopy.time_step = deepcopy(self.time_step, memo)
+        # the copy's sources must inject at the copied time: move its engine
+        # clock (zero state, just initialised) to the same step
+        newcopy._restart_clock(int(self._lib.get_step(self._sim)))
         return newcopy
+
+    def _restart_clock(self, step):
+        """Set the engine's step clock of a freshly initialised simulation
+        (its auxiliary state is still zero) to ``step``."""
+        self._push_dirty()
+        blob = self._lib.snapshot(self._sim)
+        blob[8:16].view(np.int64)[0] = step
+        self._lib.restore(self._sim, blob)
 
     def close(self):
         if self._sim is not None and self._lib is not None:

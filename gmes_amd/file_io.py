@@ -1,59 +1,57 @@
-"""Probe files and optional dumps (gmes/file_io.py:9-69).
+"""Probe files and image dumps.
 
-A ``Probe`` here writes the values the engine recorded on the device;
-the text format is the reference's (``# location=``, ``# dt=``, then
-``n value`` lines).
+The engine samples probed cells on the device every half step into a ring
+buffer; the host drains it after each run of steps and appends the
+samples to one text file per probe.  The file format is the reference's
+(gmes/file_io.py:9-48), since scripts read it: a ``# location=`` line, a
+``# dt=`` line, then one ``n value`` line per sample with the value printed
+as a numpy scalar of the field's dtype.
 """
 
-from os.path import exists
-from pathlib import Path
-from sys import stderr
+import os
+import sys
 
 import numpy as np
 
 
-class Probe(object):
-    def __init__(self, idx, field, filename):
-        self.idx = tuple(idx)
-        self.field = field
-        f_name = str(filename)
-        if exists(f_name):
-            stderr.write("Warning: " + f_name + " already exists.\n")
+class ProbeFile(object):
+    """The text file of one probe: the header at creation, then samples
+    appended in batches as they are read back from the device."""
+
+    def __init__(self, path, location, dt, dtype):
+        path = str(path)
+        self.path = path
+        self.dtype = np.dtype(dtype)
+        self.pid = None      # the engine's probe id, set by the simulation
+        # the reference's warnings, word for word (gmes/file_io.py:12-20)
+        if os.path.exists(path):
+            sys.stderr.write("Warning: " + path + " already exists.\n")
         try:
-            self.f = open(f_name, "w")
+            self._fh = open(path, "w")
         except IOError:
-            self.f = None
-            print("Warning: Can't open file " + f_name + ".\n")
+            self._fh = None
+            print("Warning: Can't open file " + path + ".\n")
+            return
+        self._fh.write("# location=%s\n# dt=%s\n" % (location, dt))
+        self._fh.flush()
+
+    def append(self, steps, values):
+        """Append samples: steps[i] is the half-step index n of values[i]."""
+        if self._fh is None or len(values) == 0:
+            return
+        vals = np.asarray(values).astype(self.dtype, copy=False)
+        self._fh.write("".join("%s %s\n" % (n, v) for n, v in zip(steps, vals)))
+        self._fh.flush()
+
+    def close(self):
+        if self._fh is not None and not self._fh.closed:
+            self._fh.close()
 
     def __del__(self):
         self.close()
 
-    def close(self):
-        if getattr(self, "f", None) is not None and not self.f.closed:
-            self.f.close()
 
-    def write_header(self, p, dt):
-        self.f.write("# location=" + str(p) + "\n")
-        self.f.write("# dt=" + str(dt) + "\n")
-
-    def write(self, n):
-        self.f.write(str(n) + " " + str(self.field[self.idx]) + "\n")
-
-    def write_value(self, n, value):
-        """One recorded sample, formatted as the reference's numpy scalar."""
-        self.f.write(str(n) + " " + str(np.asarray(value, dtype=self._dtype)[()]) + "\n")
-        self.f.flush()
-
-
-def write_hdf5(data, name, low_index, high_index):
-    from tables import open_file
-    node_name = Path(name).name
-    selection = data[low_index[0]:high_index[0], low_index[1]:high_index[1],
-                     low_index[2]:high_index[2]]
-    with open_file(name + ".h5", mode="w") as h5file:
-        h5file.create_array(h5file.root, node_name, selection)
-
-
-def snapshot(data, filename, title):
+def save_png(data, filename, title):
+    """A plane of a field as an image (needs matplotlib)."""
     import matplotlib.pyplot as plt
     plt.imsave(filename + "_" + title + ".png", np.real(data))

@@ -17,7 +17,11 @@ simulation and a result monitor, plus a step count.  The loop:
 * between chunks, retires finished requests (reads their result, frees
   their device memory) and lets newly built ones join, so requests that
   arrive while others run are admitted as slots and memory free up and
-  their work overlaps on the device.
+  their work overlaps on the device;
+* isolates failures: a request whose build raises, whose kernels report a
+  deferred error (the DM2 corrector's non-convergence, src/pw_dm2.hh:
+  398-402) or whose result cannot be read is retired with ``error`` set
+  at the next retirement point, and the others keep running.
 
 Each simulation keeps its own fields and clock, and batching does not
 change its arithmetic: a result is the same alone and in any mix.  Latency
@@ -151,20 +155,28 @@ class Server(object):
                 self.rounds += 1
                 for r in active:
                     r.remaining -= n
-                finished = [r for r in active if r.remaining == 0]
-                if finished:
+                if any(r.remaining == 0 for r in active):
                     ts = now()
-                    batch.sync()
+                    # a member's device error retires that request alone
+                    errors = batch.member_errors()
                     self.t_sync += now() - ts
-                    for r in finished:
-                        r.result = r.result_fn()
+                    for i, msg in errors.items():
+                        r = next(q for q in active if q.sim is batch.sims[i])
+                        r.error = msg
+                    retired = [r for r in active if r.remaining == 0 or r.error is not None]
+                    for r in retired:
+                        if r.error is None:
+                            try:
+                                r.result = r.result_fn()
+                            except Exception as e:  # reported on this request only
+                                r.error = repr(e)
                         r.t_done = now()
                         active.remove(r)
                         reserved -= r.footprint
                         done.append(r)
                     batch.close()
                     batch = None
-                    for r in finished:
+                    for r in retired:
                         r.sim.close()
                         r.sim = None
                     changed = True

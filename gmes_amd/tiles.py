@@ -39,6 +39,7 @@ class TileSet(object):
         self.profiles = {}         # (prof, comp, axis) -> float64 [n][w]
         self.pml_sets = {}         # id(pml) -> prof index
         self.class_counts = {}     # comp -> {cls: count}
+        self.type_counts = {}      # comp -> {reference updater prefix: count}
         self.owner_materials = {}  # comp -> {record id: material}
 
     def record_id(self, tile, pml_index):
@@ -131,6 +132,20 @@ def ghost_mask_slices(comp, shape):
     return out
 
 
+# The reference names an updater type after the material class that makes
+# it (src/material.pyx get_pw_material_<c>: DrudeHxReal, CpmlEzCmplx, ...);
+# subclasses reuse their base's updaters (DcpRc makes DcpPlrc updaters).
+_UPDATER_PREFIXES = ("Dummy", "Const", "Dielectric", "Upml", "Cpml", "DcpAde", "DcpPlrc", "Drude",
+                     "Lorentz", "Dm2")
+
+
+def updater_prefix(material):
+    for klass in type(material).__mro__:
+        if klass.__name__ in _UPDATER_PREFIXES:
+            return klass.__name__
+    return type(material).__name__
+
+
 def build_tiles(space, geom_list, e_comps, h_comps, cmplx):
     """Map every active component; returns a TileSet."""
     ts = TileSet()
@@ -164,8 +179,17 @@ def build_tiles(space, geom_list, e_comps, h_comps, cmplx):
             lut[kv] = rid
             mats[rid] = mat
         ids = lut[key]
+        ghost = np.zeros(shape, bool)
         for sl in ghost_mask_slices(comp, shape):
             ids[sl] = dummy_rid
+            ghost[sl] = True
+        net = np.bincount(key[~ghost], minlength=nkey * nkey)
+        types = {"Dummy": int(ghost.sum())} if ghost.any() else {}
+        for kv in present:
+            if net[kv]:
+                name = updater_prefix(geom_list[int(kv) // nkey].material)
+                types[name] = types.get(name, 0) + int(net[kv])
+        ts.type_counts[comp] = types
         ts.ids[comp] = np.ascontiguousarray(ids)
         cls_of = np.array([r["cls"] for r in ts.records], np.int32)
         counts = np.bincount(cls_of[ids.ravel()], minlength=9)

@@ -10,9 +10,10 @@
  *
  * Reference interfaces replaced (one C-ABI call stands for a family of
  * SWIG/Python calls of the reference):
- *   PwMaterial<T>::attach/merge      src/pw_material.hh:75-97
+ *   PwMaterial<T>::attach            src/pw_material.hh:62-64, and
+ *   PwMaterial<T>::merge             src/pw_material.hh:82-83
  *       -> gmes_sim_set_materials + gmes_sim_set_material_ids + gmes_sim_set_profile
- *   PwMaterial<T>::update_all        src/pw_material.hh:79-85 (called from
+ *   PwMaterial<T>::update_all        src/pw_material.hh:66-73 (called from
  *       FDTD.update_<c>, gmes/fdtd.py:673-809) -> gmes_sim_step
  *   FDTD.talk_with_<c>_neighbors     gmes/fdtd.py:811-1097 -> folded into gmes_sim_step
  *   PwSource.attach/update_all       gmes/pw_source.py:18-46 -> gmes_sim_set_sources,
@@ -146,7 +147,9 @@ int gmes_device_count(int32_t* count);
 
 /* ---- sim lifetime ----------------------------------------------------- */
 
-/* Allocate the fields (zeroed) on the device given by desc->device. */
+/* Allocate the fields (zeroed) on the device given by desc->device.
+ * GMES_EINVAL, before any device call, when a component would hold 2^31
+ * device elements or more (kernels use 32-bit offsets). */
 int gmes_sim_create(const gmes_grid_desc* desc, gmes_sim** sim);
 int gmes_sim_destroy(gmes_sim* sim);
 /* Shape of component comp (inactive -> 1,1,1). */
@@ -208,7 +211,14 @@ int gmes_sim_read_dft(gmes_sim* sim, int32_t id, double* out, int64_t count);
 /* Host buffers are C-order arrays of the sim's storage type. */
 int gmes_sim_write_field(gmes_sim* sim, int32_t comp, const void* host, int64_t bytes);
 int gmes_sim_read_field(gmes_sim* sim, int32_t comp, void* host, int64_t bytes);
-/* Auxiliary state of the whole sim as one opaque blob (snapshot/restore). */
+/* Everything but the sim's own six fields, as one opaque blob, for
+ * pre-emption and restart: the step clock, CPML/UPML and dispersive state,
+ * the running-DFT accumulators, and each attached auxiliary grid (its
+ * fields and its own blob).  Restore returns the clock to the snapshot's
+ * step, drops probe values not yet read and clears a deferred error; the
+ * caller restores the six fields with gmes_sim_write_field.  The
+ * reference's floor is FDTD.__deepcopy__ (gmes/fdtd.py:327-351), which
+ * keeps fields and time only. */
 int gmes_sim_state_bytes(const gmes_sim* sim, int64_t* bytes);
 int gmes_sim_snapshot(gmes_sim* sim, void* host, int64_t bytes);
 int gmes_sim_restore(gmes_sim* sim, const void* host, int64_t bytes);

@@ -292,35 +292,80 @@ int gmes_sim_read_field(gmes_sim *s, int32_t c, void *host, int64_t bytes) {
   return GMES_OK;
 }
 
-int gmes_sim_state_bytes(const gmes_sim *s, int64_t *bytes) {
-  int64_t b = 0;
+/* Snapshot blob: magic, step, the per-class state lists, the DFT
+ * accumulators, then per attached auxiliary grid its six fields and its
+ * own blob.  Restore also sets the step clock and drops unread probe
+ * values (the engine's semantics). */
+#define OSNAP_MAGIC 0x31504e5353454d47LL
+
+static int64_t oblob_bytes(const gmes_sim *s) {
+  int64_t b = 16;
   for (int c = 0; c < 6; ++c)
     for (int k = 0; k < GMES_NCLS; ++k) b += s->cls[c][k].n * s->cls[c][k].nstate * 16;
-  *bytes = b;
+  for (int d = 0; d < s->ndft; ++d) b += (int64_t)s->dft_nfreq * s->dft[d].ncells * 16;
+  for (int i = 0; i < s->naux; ++i) {
+    const gmes_sim *a = s->aux[i];
+    if (!a) continue;
+    for (int c = 0; c < 6; ++c) b += a->ncell[c] * a->lanes * 8;
+    b += oblob_bytes(a);
+  }
+  return b;
+}
+
+static void ocopy(void *dst, const void *src, size_t n) { if (n) memcpy(dst, src, n); }
+
+static int64_t oblob_copy(gmes_sim *s, char *p, int save) {
+  int64_t head[2] = {OSNAP_MAGIC, s->step};
+  if (save) memcpy(p, head, sizeof head);
+  else memcpy(head, p, sizeof head);
+  if (!save && (head[0] != OSNAP_MAGIC || head[1] < 0)) return -1;
+  int64_t done = sizeof head;
+  for (int c = 0; c < 6; ++c)
+    for (int k = 0; k < GMES_NCLS; ++k) {
+      size_t b = (size_t)(s->cls[c][k].n * s->cls[c][k].nstate * 16);
+      if (save) ocopy(p + done, s->cls[c][k].state, b); else ocopy(s->cls[c][k].state, p + done, b);
+      done += (int64_t)b;
+    }
+  for (int d = 0; d < s->ndft; ++d) {
+    size_t b = (size_t)((int64_t)s->dft_nfreq * s->dft[d].ncells * 16);
+    if (save) ocopy(p + done, s->dft[d].acc, b); else ocopy(s->dft[d].acc, p + done, b);
+    done += (int64_t)b;
+  }
+  for (int i = 0; i < s->naux; ++i) {
+    gmes_sim *a = s->aux[i];
+    if (!a) continue;
+    for (int c = 0; c < 6; ++c) {
+      size_t b = (size_t)(a->ncell[c] * a->lanes * 8);
+      if (save) ocopy(p + done, a->f[c], b); else ocopy(a->f[c], p + done, b);
+      done += (int64_t)b;
+    }
+    int64_t sub = oblob_copy(a, p + done, save);
+    if (sub < 0) return -1;
+    done += sub;
+  }
+  if (!save) {
+    s->step = head[1];
+    s->err = 0;
+    for (int i = 0; i < s->nprobe; ++i) s->probe[i].n = 0;
+  }
+  return done;
+}
+
+int gmes_sim_state_bytes(const gmes_sim *s, int64_t *bytes) {
+  *bytes = oblob_bytes(s);
   return GMES_OK;
 }
 
 int gmes_sim_snapshot(gmes_sim *s, void *host, int64_t bytes) {
-  int64_t need; gmes_sim_state_bytes(s, &need);
-  if (bytes != need) return ocore_set_error(GMES_EINVAL, "size mismatch");
-  char *p = (char *)host;
-  for (int c = 0; c < 6; ++c)
-    for (int k = 0; k < GMES_NCLS; ++k) {
-      size_t b = (size_t)(s->cls[c][k].n * s->cls[c][k].nstate * 16);
-      memcpy(p, s->cls[c][k].state, b); p += b;
-    }
+  if (bytes != oblob_bytes(s)) return ocore_set_error(GMES_EINVAL, "size mismatch");
+  oblob_copy(s, (char *)host, 1);
   return GMES_OK;
 }
 
 int gmes_sim_restore(gmes_sim *s, const void *host, int64_t bytes) {
-  int64_t need; gmes_sim_state_bytes(s, &need);
-  if (bytes != need) return ocore_set_error(GMES_EINVAL, "size mismatch");
-  const char *p = (const char *)host;
-  for (int c = 0; c < 6; ++c)
-    for (int k = 0; k < GMES_NCLS; ++k) {
-      size_t b = (size_t)(s->cls[c][k].n * s->cls[c][k].nstate * 16);
-      memcpy(s->cls[c][k].state, p, b); p += b;
-    }
+  if (bytes != oblob_bytes(s)) return ocore_set_error(GMES_EINVAL, "size mismatch");
+  if (oblob_copy(s, (char *)host, 0) < 0)
+    return ocore_set_error(GMES_EINVAL, "not a snapshot of this engine");
   return GMES_OK;
 }
 
@@ -522,7 +567,7 @@ int gmes_sim_step(gmes_sim *s, int64_t nsteps) {
   if (s->err) return s->err;
   for (int64_t t = 0; t < nsteps; ++t) {
     int rc = one_step(s);
-    if (rc) { s->err = rc; return rc; }
+    if (rc) { s->err = rc; snprintf(s->errmsg, sizeof s->errmsg, "%s", gmes_last_error()); return rc; }
   }
   return GMES_OK;
 }
@@ -567,8 +612,20 @@ int gmes_batch_create(gmes_sim *const *sims, int32_t n, gmes_batch **out) {
   return GMES_OK;
 }
 int gmes_batch_destroy(gmes_batch *b) { if (b) { free(b->sims); free(b); } return GMES_OK; }
+/* Every member is stepped; a member whose update fails (DM2) keeps its
+ * error and stops, and the error is reported by its sync, as the engine
+ * defers device errors per member to gmes_sim_sync / gmes_batch_sync. */
 int gmes_batch_step(gmes_batch *b, int64_t nsteps) {
-  for (int i = 0; i < b->n; ++i) { int rc = gmes_sim_step(b->sims[i], nsteps); if (rc) return rc; }
+  if (nsteps < 0) return ocore_set_error(GMES_EINVAL, "negative step count");
+  for (int i = 0; i < b->n; ++i) {
+    gmes_sim *s = b->sims[i];
+    if (!s->committed) return ocore_set_error(GMES_EINVAL, "sim not committed");
+  }
+  for (int i = 0; i < b->n; ++i) {
+    gmes_sim *s = b->sims[i];
+    int rc = gmes_sim_step(s, nsteps);
+    if (rc && !s->err) return rc;
+  }
   return GMES_OK;
 }
 int gmes_batch_sync(gmes_batch *b) {

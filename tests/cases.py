@@ -248,3 +248,20 @@ def gaussian_beam2d(lib, precision="fp64"):
     sim = g.TMzFDTD(space, geometry, sources, verbose=False, precision=precision, _engine=lib)
     sim.init()
     return sim
+
+
+def many_records3d(lib, precision="fp64", nobj=320, n=40, seed=7):
+    """A 3-D box with more distinct material records than a byte can index
+    (each small block has its own eps), a CPML and a pulsed Ex source."""
+    rng = np.random.default_rng(seed)
+    space = g.Cartesian(size=(n, n, n), resolution=1)
+    geometry = [g.DefaultMedium(material=g.Dielectric(1))]
+    for i in range(nobj):
+        c = rng.uniform(-n / 2 + 6, n / 2 - 6, 3)
+        geometry.append(g.Block(g.Dielectric(1.5 + 0.01 * i), center=tuple(c), size=(2, 2, 2)))
+    geometry.append(g.Shell(g.Cpml(), thickness=4))
+    src = [g.PointSource(g.Bandpass(freq=0.1, fwidth=0.05), center=(0.5, 0, 0), component=g.Ex)]
+    sim = g.FDTD(space, geometry, src, courant_ratio=0.5, verbose=False, precision=precision,
+                 _engine=lib)
+    sim.init()
+    return sim

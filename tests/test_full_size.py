@@ -75,14 +75,39 @@ def test_dispersive_cell_linear_in_amplitude(precision):
         _assert_doubled(*out, exact=False)
 
 
-def test_spheres256_fp32_tracks_fp64():
-    """FP32 against FP64 on the full 256^3 workload after 300 steps."""
+def _rel_l2(a, b):
+    num = sum(float(np.sum(np.abs(a[c].astype(np.complex128) - b[c]) ** 2)) for c in b)
+    den = sum(float(np.sum(np.abs(b[c].astype(np.complex128)) ** 2)) for c in b)
+    assert den > 0
+    return (num / den) ** 0.5
+
+
+def test_spheres256_fp32_tracks_fp64_2000_steps():
+    """FP32 against FP64 on the full 256^3 workload after its 2000 steps:
+    relative L2 over all six components <= 1e-5."""
     f = {}
     for p in ("fp32", "fp64"):
         sim = workloads.spheres3d(256, precision=p)
-        sim._advance(300)
+        sim._advance(2000)
+        assert sim._lib.get_step(sim._sim) == 2000
         f[p] = _fields(sim)
         sim.close()
-    num = sum(float(np.sum((f["fp32"][c].astype(np.float64) - f["fp64"][c]) ** 2)) for c in f["fp64"])
-    den = sum(float(np.sum(f["fp64"][c] ** 2)) for c in f["fp64"])
-    assert den > 0 and (num / den) ** 0.5 <= 1e-5
+    err = _rel_l2(f["fp32"], f["fp64"])
+    print("256^3 spheres, 2000 steps: rel L2 FP32 vs FP64 = %.3g" % err)
+    assert err <= 1e-5
+
+
+def test_dispersive_cell_c64_tracks_c128_3000_steps():
+    """Complex FP32 against complex FP64 on the full 384 x 384 x 192
+    dispersive Bloch cell (Drude, Lorentz, DCP, CPML, TFSF) after its 3000
+    steps: relative L2 over all six components <= 1e-5."""
+    f = {}
+    for p in ("fp32", "fp64"):
+        sim = workloads.dispersive3d(precision=p)
+        sim._advance(3000)
+        assert sim._lib.get_step(sim._sim) == 3000
+        f[p] = _fields(sim)
+        sim.close()
+    err = _rel_l2(f["fp32"], f["fp64"])
+    print("dispersive cell, 3000 steps: rel L2 c64 vs c128 = %.3g" % err)
+    assert err <= 1e-5

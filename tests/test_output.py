@@ -99,3 +99,38 @@ def test_probe_files_gpu_match_oracle(engine_lib, oracle_lib, tmp_path):
         out.append([open(prefix + s).read() for s in ("_ez.dat", "_hx.dat", "_hy.dat")])
     assert out[0] == out[1]
     assert len(out[0][0].splitlines()) == 2 + 38
+
+
+def _material_update_sizes(simulation):
+    """benchmarks/field_updates.py:83-96 of the reference, restated."""
+    return sorted(({"cells": u.idx_size(), "component": c.__name__, "material": type(u).__name__}
+                   for c, updaters in simulation.pw_material.items() for u in updaters.values()),
+                  key=lambda item: (item["component"], item["material"]))
+
+
+def test_pw_material_views_carry_reference_type_names(oracle_lib):
+    """tests/test_benchmarks.py:39-56 of the reference reads updater type
+    names such as DielectricEzReal and CpmlEzReal; each material class names
+    its own updaters for every component (src/material.pyx), ghost planes
+    are Dummy, and the counts cover every storage element."""
+    space = g.Cartesian(size=(4, 4, 0), resolution=5)
+    geometry = [g.DefaultMedium(g.Dielectric()),
+                g.Block(g.Drude(dps=(g.DrudePole(omega=1.0, gamma=0.1),)), center=(0.5, 0, 0),
+                        size=(1, 1, 1)),
+                g.Block(g.DcpRc(dps=(g.DrudePole(omega=1.0, gamma=0.1),),
+                                cps=(g.CriticalPoint(amp=0.5, phi=0.1, omega=1.0, gamma=0.2),)),
+                        center=(-0.8, 0, 0), size=(0.6, 0.6, 1)),
+                g.Shell(g.Cpml(), thickness=0.6)]
+    sim = g.TMzFDTD(space, geometry, [], verbose=False, _engine=oracle_lib)
+    sim.init()
+    items = _material_update_sizes(sim)
+    names = {i["material"] for i in items}
+    for want in ("DielectricEzReal", "CpmlEzReal", "DrudeEzReal", "DrudeHxReal", "DcpPlrcEzReal",
+                 "DummyHxReal", "CpmlHyReal"):
+        assert want in names, want
+    for comp in (g.Ez, g.Hx, g.Hy):
+        total = sum(i["cells"] for i in items if i["component"] == comp.__name__)
+        assert total == sim.field[comp].size
+    view = next(u for u in sim.pw_material[g.Ez].values() if type(u).__name__ == "DrudeEzReal")
+    assert view.name() == "DrudeElectric"
+    sim.close()

@@ -130,3 +130,52 @@ def test_shortest_policy_admits_least_work_first(oracle_lib):
     assert order["shortest"] == sorted(range(5), key=lambda i: steps[i])
     with pytest.raises(ValueError):
         Server(hbm_budget=1, policy="fifo")
+
+
+def dm2_request(lib, rid, rtol, steps, arrival):
+    """A 1-D two-level-medium solve; rtol < 0 can never be met, so its DM2
+    corrector reports non-convergence (tests/test_pw_dm2.py:95-101 of the
+    reference)."""
+    from cases import dm2_1d
+
+    def build():
+        sim = dm2_1d(lib, rtol=rtol)
+        return sim, lambda: np.array(sim.field[g.Ex])
+    return Request(rid, build, steps, arrival=arrival, footprint=10 ** 5)
+
+
+def _failure_mix(lib):
+    reqs = requests(lib)[:3]
+    reqs.append(dm2_request(lib, 3, -1.0, 40, 0.0))
+    reqs.append(dm2_request(lib, 4, 1e-10, 30, 0.005))
+    return reqs
+
+
+def _check_failure_isolated(lib):
+    srv = Server(hbm_budget=10 ** 10, chunk=16, workers=2)
+    done = srv.run(_failure_mix(lib))
+    assert sorted(r.rid for r in done) == list(range(5))
+    by = {r.rid: r for r in done}
+    assert by[3].error and "failed to converge" in by[3].error and by[3].result is None
+    ref = alone(lib)
+    for rid in range(3):
+        assert by[rid].error is None
+        np.testing.assert_array_equal(by[rid].result, ref[rid])
+    sim, fn = dm2_request(lib, 4, 1e-10, 30, 0.0).build()
+    sim._advance(30)
+    assert by[4].error is None
+    np.testing.assert_array_equal(by[4].result, fn())
+    sim.close()
+    stats = latency_stats(done)
+    assert stats["p50"] is not None
+
+
+def test_failing_request_is_isolated_oracle(oracle_lib):
+    """One request of a batch fails on the device: it alone is retired with
+    its error, the others complete with their alone results."""
+    _check_failure_isolated(oracle_lib)
+
+
+@pytest.mark.gpu
+def test_failing_request_is_isolated_gpu(engine_lib):
+    _check_failure_isolated(engine_lib)
