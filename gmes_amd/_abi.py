@@ -85,6 +85,7 @@ _SIGNATURES = {
     "gmes_sim_write_field": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
     "gmes_sim_read_field": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
     "gmes_sim_state_bytes": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "gmes_sim_field_sum_sq": (C.c_int, [_P, C.c_int32, _P, _P, _P]),
     "gmes_sim_snapshot": (C.c_int, [_P, _P, C.c_int64]),
     "gmes_sim_restore": (C.c_int, [_P, _P, C.c_int64]),
     "gmes_sim_step": (C.c_int, [_P, C.c_int64]),
@@ -101,6 +102,8 @@ _SIGNATURES = {
     "gmes_batch_step": (C.c_int, [_P, C.c_int64]),
     "gmes_batch_sync": (C.c_int, [_P]),
     "gmes_batch_set_graph": (C.c_int, [_P, C.c_int32]),
+    "gmes_batch_profile": (C.c_int, [_P, C.c_int32]),
+    "gmes_batch_read_profile": (C.c_int, [_P, _P, _P]),
 }
 
 EXPORTED = tuple(_SIGNATURES)
@@ -214,6 +217,14 @@ class Library(object):
         self.call("gmes_sim_state_bytes", h, C.byref(n))
         return n.value
 
+    def field_sum_sq(self, h, comp, lo, hi):
+        """sum |F|^2 over the box [lo, hi) of component comp (FP64)."""
+        lo = np.ascontiguousarray(lo, np.int64)
+        hi = np.ascontiguousarray(hi, np.int64)
+        out = np.zeros(1, np.float64)
+        self.call("gmes_sim_field_sum_sq", h, comp, _ptr(lo), _ptr(hi), _ptr(out))
+        return float(out[0])
+
     def snapshot(self, h):
         buf = np.empty(self.state_bytes(h), np.uint8)
         self.call("gmes_sim_snapshot", h, _ptr(buf), buf.nbytes)
@@ -287,6 +298,15 @@ class Library(object):
 
     def batch_sync(self, b):
         self.call("gmes_batch_sync", b)
+
+    def batch_profile(self, b, enable):
+        self.call("gmes_batch_profile", b, 1 if enable else 0)
+
+    def batch_read_profile(self, b):
+        ms = np.zeros(6, np.float64)
+        n = np.zeros(6, np.int64)
+        self.call("gmes_batch_read_profile", b, _ptr(ms), _ptr(n))
+        return ms, n
 
     def batch_set_graph(self, b, enable):
         self.call("gmes_batch_set_graph", b, int(enable))

@@ -58,6 +58,16 @@ class Batch(object):
                 out[i] = str(e)
         return out
 
+    def profile(self, enable):
+        """Time the batch's kernel families with HIP events (see
+        read_profile); steps run without the captured graph meanwhile."""
+        self._lib.batch_profile(self._b, enable)
+
+    def read_profile(self):
+        """(ms, launches) per family: E 3-D main, H 3-D main, E 2-D main,
+        H 2-D main, dispersive, the rest."""
+        return self._lib.batch_read_profile(self._b)
+
     def close(self):
         """Dissolve the batch (the members stay usable on their own)."""
         if self._b is not None:

@@ -982,6 +982,52 @@ int ctl_write(gmes_sim* s) {
   return GMES_OK;
 }
 
+// ---- sum of |F|^2 over a box
+static int sq_check(const gmes_sim* s, int32_t c, const int64_t lo[3], const int64_t hi[3], const double* out,
+                    int64_t& ncells) {
+  if (!s || !lo || !hi || !out) return set_error(GMES_EINVAL, "null argument");
+  if (c < 0 || c > 5) return set_error(GMES_EINVAL, "bad component");
+  ncells = 1;
+  for (int a = 0; a < 3; ++a) {
+    if (lo[a] < 0 || hi[a] > s->shape[c][a] || hi[a] <= lo[a]) return set_error(GMES_ERANGE, "box out of bounds");
+    ncells *= hi[a] - lo[a];
+  }
+  return GMES_OK;
+}
+
+template <class T>
+static void sq_launch(const gmes_sim* s, int c, const int64_t lo[3], const int64_t hi[3], double* partial) {
+  SqBox<T> b;
+  b.f = static_cast<const T*>(s->f[c]);
+  b.s1 = (int)s->shape[c][1];
+  b.p = s->pitch[c];
+  b.koff = s->koff[c];
+  b.ncells = 1;
+  for (int a = 0; a < 3; ++a) {
+    b.lo[a] = (int)lo[a];
+    b.n[a] = (int)(hi[a] - lo[a]);
+    b.ncells *= b.n[a];
+  }
+  launch_sum_sq<T>(b, partial, s->stream);
+}
+
+// Runs the two passes on the sim's stream into partial[0 .. nblocks] and
+// copies partial[nblocks] to *out.
+static int sq_run(gmes_sim* s, int32_t c, const int64_t lo[3], const int64_t hi[3], double* out, double* partial,
+                  int64_t ncells) {
+  HIP_TRY(sync_exec(s));  // steps of a batch or a parent write these fields on their streams
+  switch (s->g.dtype) {
+    case GMES_F32: sq_launch<float>(s, c, lo, hi, partial); break;
+    case GMES_F64: sq_launch<double>(s, c, lo, hi, partial); break;
+    case GMES_C64: sq_launch<cplx<float>>(s, c, lo, hi, partial); break;
+    default: sq_launch<cplx<double>>(s, c, lo, hi, partial); break;
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, partial + sq_blocks(ncells), sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return GMES_OK;
+}
+
 }  // namespace
 
 // =============================================================== C-ABI
@@ -1096,7 +1142,7 @@ int gmes_sim_destroy(gmes_sim* s) {
     if (d.acc) (void)dev_release(d.acc);
   void* bufs[] = {s->d_recs, s->d_coef, s->d_prof, s->d_prof_off, s->d_ops, s->d_groups, s->d_waves,
                   s->d_aux_ptrs, s->d_probe_cells, s->d_probe_comps, s->d_probe_buf, s->d_clock,
-                  s->d_dft_w, s->d_dft_jobs};
+                  s->d_dft_w, s->d_dft_jobs, s->d_sq};
   for (void* b : bufs)
     if (b) (void)dev_release(b);
   if (s->stream) (void)stream_release(s->stream);
@@ -1654,6 +1700,24 @@ int gmes_sim_restore(gmes_sim* s, const void* host, int64_t bytes) {
   return blob_copy(s, const_cast<char*>(static_cast<const char*>(host)), false, done);
 }
 
+int gmes_sim_field_sum_sq(gmes_sim* s, int32_t c, const int64_t lo[3], const int64_t hi[3], double* out) {
+  ApiGuard api;
+  int64_t ncells = 0;
+  int rc = sq_check(s, c, lo, hi, out, ncells);
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(s->device));
+  const int64_t need = sq_blocks(ncells) + 1;
+  if (need > s->sq_cap) {
+    HIP_TRY(sync_exec(s));
+    if (s->d_sq) dev_free(s, s->d_sq, (size_t)s->sq_cap * sizeof(double));
+    s->d_sq = nullptr;
+    s->sq_cap = 0;
+    if ((rc = dev_alloc(s, (void**)&s->d_sq, (size_t)need * sizeof(double)))) return rc;
+    s->sq_cap = need;
+  }
+  return sq_run(s, c, lo, hi, out, s->d_sq, ncells);
+}
+
 int gmes_sim_step(gmes_sim* s, int64_t nsteps) {
   ApiGuard api;
   if (!s) return set_error(GMES_EINVAL, "null sim handle");
@@ -1794,6 +1858,9 @@ struct gmes_batch {
   JobSet aux_set;                  // their auxiliary (TFSF) grids, stepped inside the members' step
   std::vector<gmes_sim*> aux_sims;
   std::vector<int64_t> shape_sig;  // launch shapes the graph was captured with
+  // kernel timing (gmes_batch_profile): [E 3-D main, H 3-D main, E 2-D/1-D
+  // main, H 2-D/1-D main, dispersive, the rest]
+  gmes_engine::Timing timing;
 };
 
 namespace {
@@ -1902,36 +1969,68 @@ const J* job_ptr(const gmes_batch* b, size_t off) {
 
 // One step of a group of sims (the members, or their auxiliary grids)
 // with one launch per kernel kind.
+// HIP events around the launches of one kernel family of a batch step
+// (members only; the auxiliary grids count as "the rest").
+void btime_begin(gmes_batch* b, bool on, int fam, hipStream_t st) {
+  if (!on || !b->timing.on) return;
+  hipEvent_t x, y;
+  (void)hipEventCreate(&x);
+  (void)hipEventCreate(&y);
+  (void)hipEventRecord(x, st);
+  b->timing.pending.push_back({fam, {x, y}});
+}
+
+void btime_end(gmes_batch* b, bool on, hipStream_t st) {
+  if (!on || !b->timing.on) return;
+  (void)hipEventRecord(b->timing.pending.back().second.second, st);
+}
+
 template <class T>
 void enqueue_set_step_t(gmes_batch* b, const gmes_batch::JobSet& js, const std::vector<gmes_sim*>& sims,
                         bool with_aux, hipStream_t st) {
+  const bool tm = with_aux;  // timing covers the members' launches
   const auto& H = js.ghost[1];
+  btime_begin(b, tm, 5, st);
   launch_ghost_batch<T>(job_ptr<GhostJob<T>>(b, H.off), H.njobs, H.nblocks, 1, st);
+  btime_end(b, tm, st);
   for (int e = 0; e < 2; ++e) {
     const bool electric = e == 0;
     if (!electric) {
       const auto& G = js.ghost[0];
+      btime_begin(b, tm, 5, st);
       launch_ghost_batch<T>(job_ptr<GhostJob<T>>(b, G.off), G.njobs, G.nblocks, 0, st);
+      btime_end(b, tm, st);
     }
+    btime_begin(b, tm, e, st);
     launch_main_batch<T>(electric, js.lds3d[e] ? kBatch3dLds : kBatch3d, job_ptr<MainJob<T>>(b, js.main3d[e].off),
                          js.main3d[e].njobs,
                          js.main3d[e].nblocks, st);
+    btime_end(b, tm, st);
+    btime_begin(b, tm, 2 + e, st);
     launch_main_batch<T>(electric, js.lds2z[e] ? kBatch2zLds : kBatch2z, job_ptr<MainJob<T>>(b, js.main2z[e].off),
                          js.main2z[e].njobs,
                          js.main2z[e].nblocks, st);
     launch_main_batch<T>(electric, kBatchGeneric, job_ptr<MainJob<T>>(b, js.mainnd[e].off), js.mainnd[e].njobs,
                          js.mainnd[e].nblocks, st);
-    if (electric)
+    btime_end(b, tm, st);
+    if (electric) {
+      btime_begin(b, tm, 4, st);
       for (int cls = GMES_CLS_DRUDE; cls < GMES_NCLS; ++cls)
         launch_ade_batch<T>(cls, job_ptr<AdeJob<T>>(b, js.ade[cls].off), js.ade[cls].njobs, js.ade[cls].nblocks, st);
+      btime_end(b, tm, st);
+    }
+    btime_begin(b, tm, 5, st);
     launch_src_batch<T>(job_ptr<SrcJob<T>>(b, js.src[e].off), js.src[e].njobs, js.src[e].nblocks, st);
     for (gmes_sim* s : sims) enqueue_probe<T>(s, electric, st);
     // every member's auxiliary grids advance one whole step here, all of
     // them in one launch per kernel kind (as each single sim does it with
     // its own grids)
     if (electric && with_aux && !b->aux_sims.empty()) enqueue_set_step_t<T>(b, b->aux_set, b->aux_sims, false, st);
+    btime_end(b, tm, st);
   }
+  btime_begin(b, tm, 5, st);
   launch_dft_batch<T>(job_ptr<DftJob<T>>(b, js.dft.off), js.dft.njobs, js.dft.nblocks, st);
+  btime_end(b, tm, st);
 }
 
 // One step of every member, in the single-sim order, with the main, source
@@ -2005,6 +2104,10 @@ int gmes_batch_destroy(gmes_batch* b) {
   (void)hipSetDevice(b->device);
   (void)hipStreamSynchronize(b->stream);
   destroy_batch_graph(b);
+  for (auto& p : b->timing.pending) {
+    (void)hipEventDestroy(p.second.first);
+    (void)hipEventDestroy(p.second.second);
+  }
   for (gmes_sim* s : b->sims) s->batch_stream = nullptr;
   if (b->d_jobs) (void)dev_release(b->d_jobs);
   (void)stream_release(b->stream);
@@ -2090,7 +2193,7 @@ int gmes_batch_step(gmes_batch* b, int64_t nsteps) {
   if (sig != b->shape_sig || gens != b->gens) destroy_batch_graph(b);
   b->shape_sig = sig;
   b->gens = gens;
-  if (b->use_graph) {
+  if (b->use_graph && !b->timing.on) {
     if (!b->graph_exec) {
       hipGraph_t graph;
       {
@@ -2109,6 +2212,41 @@ int gmes_batch_step(gmes_batch* b, int64_t nsteps) {
     HIP_TRY(hipGetLastError());
   }
   for (gmes_sim* s : b->sims) advance_host_clock(s, nsteps);
+  return GMES_OK;
+}
+
+int gmes_batch_profile(gmes_batch* b, int32_t enable) {
+  ApiGuard api;
+  if (!b) return set_error(GMES_EINVAL, "null batch handle");
+  HIP_TRY(hipSetDevice(b->device));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for (auto& p : b->timing.pending) {
+    (void)hipEventDestroy(p.second.first);
+    (void)hipEventDestroy(p.second.second);
+  }
+  b->timing = Timing();
+  b->timing.on = enable != 0;
+  return GMES_OK;
+}
+
+int gmes_batch_read_profile(gmes_batch* b, double ms[6], int64_t launches[6]) {
+  ApiGuard api;
+  if (!b || !ms || !launches) return set_error(GMES_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(b->device));
+  for (auto& p : b->timing.pending) {
+    float t = 0.f;
+    HIP_TRY(hipEventSynchronize(p.second.second));
+    HIP_TRY(hipEventElapsedTime(&t, p.second.first, p.second.second));
+    b->timing.ms[p.first] += t;
+    b->timing.launches[p.first] += 1;
+    (void)hipEventDestroy(p.second.first);
+    (void)hipEventDestroy(p.second.second);
+  }
+  b->timing.pending.clear();
+  for (int i = 0; i < 6; ++i) {
+    ms[i] = b->timing.ms[i];
+    launches[i] = b->timing.launches[i];
+  }
   return GMES_OK;
 }
 

@@ -162,4 +162,7 @@ struct gmes_sim {
   int tile_bx = 0, tile_by = 0, tile_ic = 0;
   gmes_engine::Timing timing;
   int64_t bytes = 0;
+  // partial sums of gmes_sim_field_sum_sq (grown on demand)
+  double* d_sq = nullptr;
+  int64_t sq_cap = 0;
 };

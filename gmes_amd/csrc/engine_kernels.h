@@ -185,6 +185,23 @@ struct AdeJob {
   int64_t block0;
 };
 
+// Sum of |F|^2 over a box of storage indices of one component, in FP64
+// and in a fixed order: pass 1 reduces kSqBlockCells cells per block into
+// partial[block]; pass 2 (one block) reduces the partials into
+// partial[nblocks].  The same box gives the same bits whatever buffer
+// holds the partials.
+constexpr int kSqBlockCells = 2048;  // 256 threads x 8 cells
+template <class T>
+struct SqBox {
+  const T* f;    // element 0 of the device array
+  int s1, p, koff;
+  int lo[3], n[3];
+  int ncells;    // < 2^31 (a component holds < 2^31 elements)
+};
+inline int64_t sq_blocks(int64_t ncells) { return (ncells + kSqBlockCells - 1) / kSqBlockCells; }
+template <class T>
+void launch_sum_sq(const SqBox<T>& b, double* partial, hipStream_t st);
+
 template <class T>
 void launch_main(bool electric, const MainArgs<T>& m, int ext[3], int bx, int by, hipStream_t st);
 // collapsed-z 2-D kernels: i rows per thread (1 measured fastest of 1, 2

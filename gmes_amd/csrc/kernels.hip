@@ -1461,7 +1461,55 @@ void launch_ghost(const GhostArgs<T>& m, int64_t total, hipStream_t st) {
   hipLaunchKernelGGL((k_ghost<T>), dim3((unsigned)blocks), dim3(256), 0, st, m);
 }
 
+// ---- sum of |F|^2 over a box (two passes, fixed order)
+__device__ __forceinline__ double sq_of(float x) { return (double)x * (double)x; }
+__device__ __forceinline__ double sq_of(double x) { return x * x; }
+template <class R>
+__device__ __forceinline__ double sq_of(cplx<R> x) { return sq_of(x.re) + sq_of(x.im); }
+
+__device__ __forceinline__ double block_sum_256(double v) {
+  __shared__ double red[256];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + w];
+    __syncthreads();
+  }
+  return red[0];
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_sum_sq(SqBox<T> b, double* __restrict__ partial) {
+  const int base = (int)blockIdx.x * kSqBlockCells;
+  double acc = 0.0;
+  for (int r = 0; r < kSqBlockCells / 256; ++r) {
+    const int q = base + r * 256 + (int)threadIdx.x;
+    if (q < b.ncells) {
+      const int k = q % b.n[2], t = q / b.n[2];
+      const int j = t % b.n[1], i = t / b.n[1];
+      acc = acc + sq_of(b.f[kPad + ((i + b.lo[0]) * b.s1 + (j + b.lo[1])) * b.p + (k + b.lo[2]) - b.koff]);
+    }
+  }
+  const double tot = block_sum_256(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(256) k_sum_partials(double* __restrict__ partial, int nblocks) {
+  double acc = 0.0;
+  for (int q = (int)threadIdx.x; q < nblocks; q += 256) acc = acc + partial[q];
+  const double tot = block_sum_256(acc);
+  if (threadIdx.x == 0) partial[nblocks] = tot;
+}
+
+template <class T>
+void launch_sum_sq(const SqBox<T>& b, double* partial, hipStream_t st) {
+  const int nb = (int)sq_blocks(b.ncells);
+  hipLaunchKernelGGL((k_sum_sq<T>), dim3(nb), dim3(256), 0, st, b, partial);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, partial, nb);
+}
+
 #define GMES_INSTANTIATE(T)                                                                     \
+  template void launch_sum_sq<T>(const SqBox<T>&, double*, hipStream_t);                        \
   template void launch_main<T>(bool, const MainArgs<T>&, int[3], int, int, hipStream_t);       \
   template void launch_main3d<T>(bool, const MainArgs<T>&, int, int, int, hipStream_t);        \
   template void launch_main2z<T>(bool, const MainArgs<T>&, int[3], hipStream_t);               \

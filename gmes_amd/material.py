@@ -456,8 +456,11 @@ class Lorentz(Dielectric, _DispersiveH):
     def _tile(self, electric, underneath, cmplx):
         if not electric:
             return self._tile_h(underneath)
+        # n2 = 1 marks a Lorentz medium: the same update as Drude
+        # (src/pw_lorentz.hh:96-107, src/pw_drude.hh:90-101); the flag names
+        # the reference's updater type for the reference-updater harness
         return dict(cls=CLS_DRUDE, eps_mu=_eps_mu(self, True, underneath), n1=len(self.lps),
-                    coef=list(self.a.ravel()) + list(self.c))
+                    n2=1, coef=list(self.a.ravel()) + list(self.c))
 
 
 class Dm2(Dielectric, _DispersiveH):

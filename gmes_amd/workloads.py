@@ -79,9 +79,10 @@ def _random_material(rng):
     return _mat.Lorentz(eps_inf=float(rng.uniform(1.0, 4.0)), lps=poles)
 
 
-def batch_instance(rng, three_d, side, precision="fp32", _engine=None, res=16):
+def batch_instance(rng, three_d, side, precision="fp32", _engine=None, res=16, init=True):
     """One instance of the batched workload: a box of ``side`` cells per
-    axis (2-D TMz or 3-D) at resolution ``res``."""
+    axis (2-D TMz or 3-D) at resolution ``res``.  init=False returns it
+    unallocated (the random draws are the same)."""
     L = side / float(res)
     pml = int(rng.integers(8, 17)) / float(res)
     inner = L / 2.0 - pml
@@ -108,7 +109,8 @@ def batch_instance(rng, three_d, side, precision="fp32", _engine=None, res=16):
     src = [_src.PointSource(_src.Bandpass(freq=f, fwidth=f / 4), center=pos, component=Ez)]
     cls = _fdtd.FDTD if three_d else _fdtd.TMzFDTD
     sim = cls(space, geom, src, courant_ratio=0.5, verbose=False, precision=precision, _engine=_engine)
-    sim.init()
+    if init:
+        sim.init()
     return sim
 
 
@@ -120,11 +122,36 @@ def batch_sizes(n2d=32, n3d=32, seed=4, lo2=128, hi2=1024, lo3=64, hi3=192):
     return out
 
 
-def batch_instances(n2d=32, n3d=32, seed=4, precision="fp32", _engine=None, **sizes):
-    """The instances of the batched workload (seeded; list order = draw order)."""
+def batch_instances(n2d=32, n3d=32, seed=4, precision="fp32", _engine=None, only=None, **sizes):
+    """The instances of the batched workload (seeded; list order = draw order).
+    only: indices of the instances to return (the others are drawn and
+    dropped, so each instance is the same whatever is selected)."""
     rng = np.random.default_rng(seed + 1000)
-    return [batch_instance(rng, t, s, precision=precision, _engine=_engine)
-            for t, s in batch_sizes(n2d, n3d, seed, **sizes)]
+    out = []
+    for i, (t, s) in enumerate(batch_sizes(n2d, n3d, seed, **sizes)):
+        keep = only is None or i in only
+        sim = batch_instance(rng, t, s, precision=precision, _engine=_engine, init=keep)
+        if keep:
+            out.append(sim)
+    return out
+
+
+def z_cpml_shell(lx, ly, lz, d):
+    """A CPML of thickness d on the two z faces of an lx x ly x lz cell
+    that is periodic along x and y.
+
+    The reference grades a PML material along every axis of its shell,
+    whichever faces the shell has (src/pygeom.pyx:1061-1079 hands the
+    shell's half sizes to Pml.init, src/material.pyx:441-470), so a z-only
+    boundary shell also carries x and y grading in the corners of its z
+    slabs next to the periodic seams.  Those unterminated patches support
+    a growing surface mode on the slab interfaces (the fields of a point
+    source in such a cell grow about 1e5-fold per 300 steps in FP64).  This
+    shell is wider than the cell along x and y by more than 2 d, so that
+    grading lies outside the grid and only the z grading acts, within the
+    reference's own API."""
+    return _geo.Shell(material=_mat.Cpml(), size=(lx + 2.5 * d, ly + 2.5 * d, lz), thickness=d,
+                      plus_x=False, minus_x=False, plus_y=False, minus_y=False)
 
 
 def _tables_of(sim_args):
@@ -157,8 +184,7 @@ def pc_slab(side, radius, steps=None, precision="fp32", res=16, nfreq=50, decima
             if abs(x) <= L / 2 + radius and abs(y) <= L / 2 + radius:
                 geom.append(_pg.Cylinder(_mat.Dielectric(1.0), center=(x, y, 0), radius=radius, height=0.5,
                                          axis=(0, 0, 1)))
-    geom.append(_geo.Shell(material=_mat.Cpml(), thickness=1.0, plus_x=False, minus_x=False,
-                           plus_y=False, minus_y=False))
+    geom.append(z_cpml_shell(L, L, L, 1.0))
     # TFSF z faces between the CPML layers: low edge on a node, high edge
     # half way between nodes (the rounding rule of the face index ranges)
     zlo = -L / 2 + 1.25 + 0.0 * dz
@@ -268,8 +294,7 @@ def dispersive3d(scale=1.0, seed=3, precision="fp32", _engine=None, courant=0.3,
                       cps=(_mat.CriticalPoint(amp=0.8, phi=-0.4, omega=0.6, gamma=0.1),
                            _mat.CriticalPoint(amp=0.4, phi=0.7, omega=0.9, gamma=0.2)))
     geom.append(_pg.Block(dcp, center=(0, 0, -zin + 8 * scale + 4), size=(2 * nx, 2 * ny, 16 * scale)))
-    geom.append(_geo.Shell(material=_mat.Cpml(), thickness=max(1.0, round(10 * scale)), plus_x=False,
-                           minus_x=False, plus_y=False, minus_y=False))
+    geom.append(z_cpml_shell(nx, ny, nz, max(1.0, round(10 * scale))))
     ztop = zin + 2
     st = _src.Bandpass(freq=0.05, fwidth=0.025) if src_time is None else src_time
     src = [_src.TotalFieldScatteredField(st, center=(0, 0, 0.25),

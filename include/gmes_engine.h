@@ -106,7 +106,7 @@ typedef struct {
 /* One material record.  Coefficients are FP64, computed on the host by the
  * reference's setup formulas (src/material.pyx).  coef_off indexes the
  * coefficient pool passed with the table; its layout per class:
- *   DRUDE   : a[n1][3], c[3]
+ *   DRUDE   : a[n1][3], c[3]; n2 = 1 marks a Lorentz medium (same update)
  *   DCP_ADE : a[n1][3], b[n2][5], c[4]
  *   DCP_PLRC: a[n1][3], b[n2][3] as (re,im) pairs, c[3]
  *   DM2     : omega[n1], n_atom[n1], rho30, gamma, t1, t2, hbar, rtol
@@ -223,6 +223,15 @@ int gmes_sim_state_bytes(const gmes_sim* sim, int64_t* bytes);
 int gmes_sim_snapshot(gmes_sim* sim, void* host, int64_t bytes);
 int gmes_sim_restore(gmes_sim* sim, const void* host, int64_t bytes);
 
+/* Sum of |F|^2 over the box [lo, hi) of storage indices of component comp,
+ * accumulated in FP64 in a fixed order (the same box gives the same bits),
+ * into *out; waits for the steps enqueued before it.  Partial sums go to a
+ * device buffer of the sim, grown on demand.  It stands for the
+ * reference's host-side checks on the NumPy arrays, such as the regression
+ * value sum(|ez|^2) of tests/test_fdtd.py:75-79, without a download. */
+int gmes_sim_field_sum_sq(gmes_sim* sim, int32_t comp, const int64_t lo[3], const int64_t hi[3],
+                          double* out);
+
 /* ---- time stepping ---------------------------------------------------- */
 
 /* Enqueue nsteps full leapfrog steps on the sim's stream (asynchronous). */
@@ -275,6 +284,12 @@ int gmes_batch_destroy(gmes_batch* batch);
 int gmes_batch_step(gmes_batch* batch, int64_t nsteps);
 int gmes_batch_sync(gmes_batch* batch);
 int gmes_batch_set_graph(gmes_batch* batch, int32_t enable);
+/* Kernel timing of a batch's own stream (HIP events; steps run without the
+ * graph while it is on): read returns ms and launch counts for
+ * [E 3-D main, H 3-D main, E 2-D/1-D main, H 2-D/1-D main, dispersive,
+ * the rest (ghosts, sources, probes, auxiliary grids, DFT)]. */
+int gmes_batch_profile(gmes_batch* batch, int32_t enable);
+int gmes_batch_read_profile(gmes_batch* batch, double ms[6], int64_t launches[6]);
 
 #ifdef __cplusplus
 }

@@ -292,6 +292,25 @@ int gmes_sim_read_field(gmes_sim *s, int32_t c, void *host, int64_t bytes) {
   return GMES_OK;
 }
 
+/* Sum of |F|^2 over a box, FP64, in C order (the engine reduces in a
+ * different order: agreement is to rounding). */
+int gmes_sim_field_sum_sq(gmes_sim *s, int32_t c, const int64_t lo[3], const int64_t hi[3], double *out) {
+  if (!s || !lo || !hi || !out) return ocore_set_error(GMES_EINVAL, "null argument");
+  if (c < 0 || c > 5) return ocore_set_error(GMES_EINVAL, "bad component");
+  for (int a = 0; a < 3; ++a)
+    if (lo[a] < 0 || hi[a] > s->shape[c][a] || hi[a] <= lo[a])
+      return ocore_set_error(GMES_ERANGE, "box out of bounds");
+  double acc = 0.0;
+  for (int64_t i = lo[0]; i < hi[0]; ++i)
+    for (int64_t j = lo[1]; j < hi[1]; ++j)
+      for (int64_t k = lo[2]; k < hi[2]; ++k) {
+        ocv v = ocore_get(s, c, i, j, k);
+        acc += v.re * v.re + v.im * v.im;
+      }
+  *out = acc;
+  return GMES_OK;
+}
+
 /* Snapshot blob: magic, step, the per-class state lists, the DFT
  * accumulators, then per attached auxiliary grid its six fields and its
  * own blob.  Restore also sets the step clock and drops unread probe
@@ -633,3 +652,9 @@ int gmes_batch_sync(gmes_batch *b) {
   return GMES_OK;
 }
 int gmes_batch_set_graph(gmes_batch *b, int32_t enable) { (void)b; (void)enable; return GMES_OK; }
+int gmes_batch_profile(gmes_batch *b, int32_t enable) { (void)b; (void)enable; return GMES_OK; }
+int gmes_batch_read_profile(gmes_batch *b, double ms[6], int64_t launches[6]) {
+  (void)b;
+  for (int i = 0; i < 6; ++i) { ms[i] = 0; launches[i] = 0; }
+  return GMES_OK;
+}

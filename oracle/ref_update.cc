@@ -9,9 +9,9 @@
  * parameter struct the reference's factories would build
  * (src/material.pyx get_pw_material_<c>), and obackend_update() calls
  * update_all() through the virtual base, as FDTD.update_<c> does
- * (gmes/fdtd.py:673-809).  DrudeEx also serves Lorentz materials: the two
- * updaters are the same recursion (src/pw_drude.hh:90-101 and
- * src/pw_lorentz.hh:96-107).
+ * (gmes/fdtd.py:673-809).  Records of class DRUDE with n2 == 1 are
+ * Lorentz media and go to the reference's LorentzE* updaters
+ * (src/pw_lorentz.hh), the others to DrudeE* (src/pw_drude.hh).
  */
 #include <complex>
 #include <memory>
@@ -24,6 +24,7 @@
 #include "pw_dielectric.hh"
 #include "pw_dm2.hh"
 #include "pw_drude.hh"
+#include "pw_lorentz.hh"
 #include "pw_upml.hh"
 
 #include "oracle_core.h"
@@ -33,6 +34,7 @@ namespace {
 template <typename T>
 struct RefSet {
   std::unique_ptr<gmes::PwMaterial<T>> up[6][GMES_NCLS];
+  std::unique_ptr<gmes::PwMaterial<T>> lorentz[6];  // DRUDE records with n2 == 1
 };
 
 template <typename T>
@@ -73,6 +75,13 @@ gmes::PwMaterial<T>* new_updater(int cls, int c) {
   default:
     return nullptr;
   }
+}
+
+template <typename T>
+gmes::PwMaterial<T>* new_lorentz(int c) {
+  using namespace gmes;
+  switch (c) { case 0: return new LorentzEx<T>(); case 1: return new LorentzEy<T>(); case 2: return new LorentzEz<T>();
+               default: return nullptr; }
 }
 
 template <typename T>
@@ -134,6 +143,14 @@ void attach_cell(gmes_sim* s, gmes::PwMaterial<T>* up, int cls, int c, int64_t x
     break;
   }
   case GMES_CLS_DRUDE: {
+    if (r.n2 == 1) {
+      LorentzElectricParam<T> p; p.eps_inf = r.eps_mu;
+      for (int q = 0; q < r.n1; ++q) p.a.push_back({cf[3 * q], cf[3 * q + 1], cf[3 * q + 2]});
+      p.c = {cf[3 * r.n1], cf[3 * r.n1 + 1], cf[3 * r.n1 + 2]};
+      p.l_now.assign(r.n1, T(0)); p.l_new.assign(r.n1, T(0));
+      up->attach(idx, 3, &p);
+      break;
+    }
     DrudeElectricParam<T> p; p.eps_inf = r.eps_mu;
     for (int q = 0; q < r.n1; ++q) p.a.push_back({cf[3 * q], cf[3 * q + 1], cf[3 * q + 2]});
     p.c = {cf[3 * r.n1], cf[3 * r.n1 + 1], cf[3 * r.n1 + 2]};
@@ -198,7 +215,12 @@ int commit_t(gmes_sim* s) {
       gmes::PwMaterial<T>* up = cls == GMES_CLS_DM2 ? new_dm2<T>(c) : new_updater<T>(cls, c);
       if (!up) return ocore_set_error(GMES_EINVAL, "class %d has no updater for component %d", cls, c);
       set->up[c][cls].reset(up);
-      for (int64_t q = 0; q < L->n; ++q) attach_cell<T>(s, up, cls, c, L->cell[q], s->recs[L->rec[q]]);
+      if (cls == GMES_CLS_DRUDE) set->lorentz[c].reset(new_lorentz<T>(c));
+      for (int64_t q = 0; q < L->n; ++q) {
+        const gmes_mat_rec& r = s->recs[L->rec[q]];
+        gmes::PwMaterial<T>* target = (cls == GMES_CLS_DRUDE && r.n2 == 1) ? set->lorentz[c].get() : up;
+        attach_cell<T>(s, target, cls, c, L->cell[q], r);
+      }
     }
   }
   return GMES_OK;
@@ -213,9 +235,9 @@ int update_t(gmes_sim* s, int c, double n) {
   const T* B = reinterpret_cast<const T*>(s->f[in2]);
   const double d1 = ocore_spacing(s, GMES_AX1[c]), d2 = ocore_spacing(s, GMES_AX2[c]);
   try {
-    for (int cls = 1; cls < GMES_NCLS; ++cls) {
-      gmes::PwMaterial<T>* up = set->up[c][cls].get();
-      if (!up) continue;
+    for (int cls = 1; cls <= GMES_NCLS; ++cls) {
+      gmes::PwMaterial<T>* up = cls < GMES_NCLS ? set->up[c][cls].get() : set->lorentz[c].get();
+      if (!up || up->idx_size() == 0) continue;
       up->update_all(F, (int)s->shape[c][0], (int)s->shape[c][1], (int)s->shape[c][2],
                      A, (int)s->shape[in1][0], (int)s->shape[in1][1], (int)s->shape[in1][2],
                      B, (int)s->shape[in2][0], (int)s->shape[in2][1], (int)s->shape[in2][2],

@@ -108,3 +108,21 @@ def test_error_codes_map_to_reference_exceptions(oracle_lib):
     ops["cell"] = 10 ** 9
     with pytest.raises(IndexError):
         oracle_lib.set_sources(sim._sim, ops, 0)
+
+
+def test_field_sum_sq_reproduces_the_tmz_regression_value(oracle_lib):
+    """gmes_sim_field_sum_sq over the whole Ez array is the reference's
+    regression value sum(|ez|^2) = 1.5780099423691636 after 5 steps
+    (tests/test_fdtd.py:55-79), and a sub-box equals numpy's sum."""
+    from cases import kat_tmz
+    sim = kat_tmz(oracle_lib)
+    for _ in range(5):
+        sim.step()
+    ez = np.array(sim.ez)
+    full = oracle_lib.field_sum_sq(sim._sim, 2, (0, 0, 0), ez.shape)
+    assert abs(full - 1.5780099423691636) < 1e-14
+    part = oracle_lib.field_sum_sq(sim._sim, 2, (2, 3, 0), (9, 7, 1))
+    assert abs(part - float(np.sum(ez[2:9, 3:7] ** 2))) <= 1e-15 * part
+    with pytest.raises(IndexError):
+        oracle_lib.field_sum_sq(sim._sim, 2, (0, 0, 0), (12, 11, 1))
+    sim.close()
