@@ -86,6 +86,7 @@ _SIGNATURES = {
     "gmes_sim_read_field": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
     "gmes_sim_state_bytes": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "gmes_sim_field_sum_sq": (C.c_int, [_P, C.c_int32, _P, _P, _P]),
+    "gmes_sim_field_sum_sq_noalloc": (C.c_int, [_P, C.c_int32, _P, _P, _P]),
     "gmes_sim_snapshot": (C.c_int, [_P, _P, C.c_int64]),
     "gmes_sim_restore": (C.c_int, [_P, _P, C.c_int64]),
     "gmes_sim_step": (C.c_int, [_P, C.c_int64]),
@@ -217,12 +218,14 @@ class Library(object):
         self.call("gmes_sim_state_bytes", h, C.byref(n))
         return n.value
 
-    def field_sum_sq(self, h, comp, lo, hi):
-        """sum |F|^2 over the box [lo, hi) of component comp (FP64)."""
+    def field_sum_sq(self, h, comp, lo, hi, allocate=True):
+        """sum |F|^2 over the box [lo, hi) of component comp (FP64);
+        allocate=False uses the no-allocation entry point."""
         lo = np.ascontiguousarray(lo, np.int64)
         hi = np.ascontiguousarray(hi, np.int64)
         out = np.zeros(1, np.float64)
-        self.call("gmes_sim_field_sum_sq", h, comp, _ptr(lo), _ptr(hi), _ptr(out))
+        name = "gmes_sim_field_sum_sq" if allocate else "gmes_sim_field_sum_sq_noalloc"
+        self.call(name, h, comp, _ptr(lo), _ptr(hi), _ptr(out))
         return float(out[0])
 
     def snapshot(self, h):

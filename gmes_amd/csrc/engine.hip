@@ -24,6 +24,11 @@
 
 using namespace gmes_engine;
 
+// Partial sums of gmes_sim_field_sum_sq_noalloc: 65536 blocks of 2048
+// cells (boxes of up to 2^27 cells) and the total, 512 KiB per GPU.
+constexpr int64_t kScratchPartials = 65536;
+static __device__ __align__(256) double scratch[kScratchPartials + 1];
+
 namespace {
 
 thread_local char g_err[512];
@@ -983,6 +988,15 @@ int ctl_write(gmes_sim* s) {
 }
 
 // ---- sum of |F|^2 over a box
+// No-allocation variant: the partial sums live in this file-scope device
+// array (one copy per GPU, held until the process ends).  Host code never
+// passes the array's name to a kernel or copy: it uses the device address
+// that hipGetSymbolAddress returns, looked up once per device with that
+// device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu[kMaxDevices];   // serialises no-allocation calls per device
+double* g_scratch_ptr[kMaxDevices] = {};
+
 static int sq_check(const gmes_sim* s, int32_t c, const int64_t lo[3], const int64_t hi[3], const double* out,
                     int64_t& ncells) {
   if (!s || !lo || !hi || !out) return set_error(GMES_EINVAL, "null argument");
@@ -1716,6 +1730,27 @@ int gmes_sim_field_sum_sq(gmes_sim* s, int32_t c, const int64_t lo[3], const int
     s->sq_cap = need;
   }
   return sq_run(s, c, lo, hi, out, s->d_sq, ncells);
+}
+
+int gmes_sim_field_sum_sq_noalloc(gmes_sim* s, int32_t c, const int64_t lo[3], const int64_t hi[3], double* out) {
+  ApiGuard api;
+  int64_t ncells = 0;
+  int rc = sq_check(s, c, lo, hi, out, ncells);
+  if (rc) return rc;
+  if (sq_blocks(ncells) > kScratchPartials)
+    return set_error(GMES_EINVAL, "box of %lld cells exceeds the no-allocation limit of %lld cells",
+                     (long long)ncells, (long long)(kScratchPartials * kSqBlockCells));
+  if (s->device < 0 || s->device >= kMaxDevices) return set_error(GMES_EINVAL, "device ordinal beyond %d", kMaxDevices);
+  HIP_TRY(hipSetDevice(s->device));
+  std::lock_guard<std::mutex> lk(g_scratch_mu[s->device]);
+  if (!g_scratch_ptr[s->device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(GMES_EDEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[s->device] = static_cast<double*>(p);
+  }
+  return sq_run(s, c, lo, hi, out, g_scratch_ptr[s->device], ncells);
 }
 
 int gmes_sim_step(gmes_sim* s, int64_t nsteps) {

@@ -231,6 +231,18 @@ int gmes_sim_restore(gmes_sim* sim, const void* host, int64_t bytes);
  * value sum(|ez|^2) of tests/test_fdtd.py:75-79, without a download. */
 int gmes_sim_field_sum_sq(gmes_sim* sim, int32_t comp, const int64_t lo[3], const int64_t hi[3],
                           double* out);
+/* gmes_sim_field_sum_sq without any device allocation, for callers that
+ * cannot allocate (a serving loop whose other requests keep the GPU busy:
+ * hipMalloc and hipFree wait for them; code that must not allocate).  No
+ * counterpart in the reference.  The partial sums go to one static device
+ * array per GPU (512 KiB, held until the process ends); boxes of more than
+ * 2^27 cells are refused with GMES_EINVAL.  Calls on one GPU share that
+ * array: the library serialises them (each returns after its result is
+ * copied out), and the array keeps the last call's partial sums; every
+ * call writes all the partials it reads, so nothing needs clearing.  The
+ * result is bit-identical to gmes_sim_field_sum_sq's. */
+int gmes_sim_field_sum_sq_noalloc(gmes_sim* sim, int32_t comp, const int64_t lo[3],
+                                  const int64_t hi[3], double* out);
 
 /* ---- time stepping ---------------------------------------------------- */
 

@@ -311,6 +311,11 @@ int gmes_sim_field_sum_sq(gmes_sim *s, int32_t c, const int64_t lo[3], const int
   return GMES_OK;
 }
 
+int gmes_sim_field_sum_sq_noalloc(gmes_sim *s, int32_t c, const int64_t lo[3], const int64_t hi[3],
+                                  double *out) {
+  return gmes_sim_field_sum_sq(s, c, lo, hi, out);
+}
+
 /* Snapshot blob: magic, step, the per-class state lists, the DFT
  * accumulators, then per attached auxiliary grid its six fields and its
  * own blob.  Restore also sets the step clock and drops unread probe

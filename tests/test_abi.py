@@ -121,6 +121,7 @@ def test_field_sum_sq_reproduces_the_tmz_regression_value(oracle_lib):
     ez = np.array(sim.ez)
     full = oracle_lib.field_sum_sq(sim._sim, 2, (0, 0, 0), ez.shape)
     assert abs(full - 1.5780099423691636) < 1e-14
+    assert oracle_lib.field_sum_sq(sim._sim, 2, (0, 0, 0), ez.shape, allocate=False) == full
     part = oracle_lib.field_sum_sq(sim._sim, 2, (2, 3, 0), (9, 7, 1))
     assert abs(part - float(np.sum(ez[2:9, 3:7] ** 2))) <= 1e-15 * part
     with pytest.raises(IndexError):
