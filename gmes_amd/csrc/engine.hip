@@ -213,6 +213,33 @@ int32_t zero_c_run(const gmes_sim* s, int p, int axis) {
   return lo | (hi << 16);
 }
 
+// Real-FP32 3-D sims whose three components of a field type all have CPML
+// cells: each axis is read by two of them (E: x by Ey and Ez, y by Ex and
+// Ez, z by Ex and Ey; H: x by Hy and Hz, y by Hx and Hz, z by Hx and Hy),
+// at the same coordinates.  When their set-0 rows are identical, and every
+// row is (b, 0, 1) inside the zero-c run of its axis, the kernels load one
+// row per axis with the cell's other operands (MainArgs::pp) instead of
+// two rows per component after the cell's record.
+void profile_prefetch(gmes_sim* s) {
+  s->pp_ok[0] = s->pp_ok[1] = false;
+  if (!is3d(s) || s->g.dtype != GMES_F32 || s->nprof < 1) return;
+  for (int e = 0; e < 2; ++e) {
+    const int c0 = e ? 3 : 0;
+    bool ok = true;
+    for (int c = c0; c < c0 + 3 && ok; ++c) ok = s->pml_state[c] && s->has_cls[c][GMES_CLS_CPML];
+    const int pairs[3][2] = {{c0 + 1, c0 + 2}, {c0, c0 + 2}, {c0, c0 + 1}};
+    for (int a = 0; a < 3 && ok; ++a) {
+      const std::vector<double>& r0 = s->prof_host[0][pairs[a][0]][a];
+      const std::vector<double>& r1 = s->prof_host[0][pairs[a][1]][a];
+      ok = s->prof_w[0][pairs[a][0]][a] == 3 && s->prof_w[0][pairs[a][1]][a] == 3 && !r0.empty() && r0 == r1;
+      const int lo = s->prun[a] & 0xFFFF, hi = s->prun[a] >> 16;
+      for (int q = lo; ok && q < hi && 3 * (size_t)q + 2 < r0.size(); ++q)
+        ok = r0[3 * (size_t)q + 1] == 0.0 && r0[3 * (size_t)q + 2] == 1.0;
+    }
+    s->pp_ok[e] = ok;
+  }
+}
+
 // Real-FP32 3-D sims: the components with UPML (and no CPML) state whose
 // state word the 3-D kernels load together with the cell's other operands,
 // and a box of storage coordinates (per axis lo | hi << 16) holding no UPML
@@ -467,6 +494,15 @@ bool main_args(const gmes_sim* s, bool electric, MainArgs<T>& m, int ext[3]) {
   }
   for (int c = 0; c < 6; ++c)
     for (int a = 0; a < 3; ++a) m.po0[c][a] = (int)s->prof_off0[c][a];
+  m.pp = 0;
+  if (s->pp_ok[electric ? 0 : 1]) {
+    // axis x: 2nd component (Ey / Hy), axis y and z: 1st component (Ex / Hx)
+    const int c0 = electric ? 0 : 3;
+    m.pp = 1;
+    m.pprow[0] = m.po0[c0 + 1][0];
+    m.pprow[1] = m.po0[c0][1];
+    m.pprow[2] = m.po0[c0][2];
+  }
   m.narrow = 1;
   for (int c = 0; c < 6; ++c) {
     m.ids[c] = s->ids[c];
@@ -1363,6 +1399,7 @@ int gmes_sim_commit(gmes_sim* s) {
     }
     s->prun[a] = (any && lo < hi) ? (lo | (hi << 16)) : 0;
   }
+  profile_prefetch(s);
   upml_box(s);
   s->committed = true;
   destroy_graph(s);
@@ -1519,7 +1556,8 @@ int gmes_sim_add_dft(gmes_sim* s, int32_t c, const int64_t lo[3], const int64_t 
   d.decim = decim;
   HIP_TRY(hipSetDevice(s->device));
   HIP_TRY(sync_exec(s));
-  const size_t bytes = (size_t)nfreq * d.ncells * 2 * sizeof(double);
+  // sums [nfreq][ncells][re, im], then their compensations, same layout
+  const size_t bytes = 2 * (size_t)nfreq * d.ncells * 2 * sizeof(double);
   HIP_TRY(hipMalloc((void**)&d.acc, bytes));
   HIP_TRY(hipMemset(d.acc, 0, bytes));
   s->dfts.push_back(d);
@@ -1623,7 +1661,7 @@ static void state_blocks(gmes_sim* s, std::vector<std::pair<void*, size_t>>& out
     for (int k = GMES_CLS_DRUDE; k < GMES_NCLS; ++k)
       if (s->ade[c][k].n) out.push_back({s->ade[c][k].state, (size_t)s->ade[c][k].nstate * s->ade[c][k].n * ade_word(s, k)});
   }
-  for (auto& d : s->dfts) out.push_back({d.acc, (size_t)d.nfreq * d.ncells * 2 * sizeof(double)});
+  for (auto& d : s->dfts) out.push_back({d.acc, 2 * (size_t)d.nfreq * d.ncells * 2 * sizeof(double)});
 }
 
 static int64_t blob_bytes(gmes_sim* s) {
@@ -1884,6 +1922,7 @@ struct gmes_batch {
   // the launches of one step of a group of sims
   struct JobSet {
     Launch main3d[2], main2z[2], mainnd[2], src[2], ghost[2];  // [0] E, [1] H
+    Launch main3d_pp[2];  // 3-D jobs with the CPML row prefetch (kBatch3dPP)
     bool lds3d[2] = {false, false};  // every 3-D job stages its records in LDS
     bool lds2z[2] = {false, false};  // every collapsed-z job does
     Launch ade[GMES_NCLS];                                      // by dispersive class
@@ -1913,8 +1952,8 @@ template <class T>
 void build_jobs(gmes_batch* b, const std::vector<gmes_sim*>& sims, gmes_batch::JobSet& out) {
   for (int e = 0; e < 2; ++e) {
     const bool electric = e == 0;
-    std::vector<MainJob<T>> j3, j2, jn;
-    int64_t nb3 = 0, nb2 = 0, nbn = 0;
+    std::vector<MainJob<T>> j3, j2, jn, j3p;
+    int64_t nb3 = 0, nb2 = 0, nbn = 0, nb3p = 0;
     bool lds3d = true, lds2z = true;
     // the block-wide record staging of the main tiles (RecStage::use_lds)
     constexpr int chunks = (int)(sizeof(DevRec<typename real_of<T>::type>) / 16);
@@ -1931,12 +1970,19 @@ void build_jobs(gmes_batch* b, const std::vector<gmes_sim*>& sims, gmes_batch::J
         J.bx = bx;
         const int tby = 256 / bx;
         if (is3d(s)) {
-          if (J.m.nrec > kRecLds || J.m.nrec * chunks > 256) lds3d = false;
+          const bool fits = J.m.nrec <= kRecLds && J.m.nrec * chunks <= 256;
           J.gx = (s->g.nz + bx - 1) / bx;
           J.gy = (s->g.ny + tby - 1) / tby;
-          J.block0 = nb3;
-          nb3 += (int64_t)J.gx * J.gy * s->g.nx;
-          j3.push_back(J);
+          if (std::is_same<T, float>::value && J.m.pp && fits) {
+            J.block0 = nb3p;
+            nb3p += (int64_t)J.gx * J.gy * s->g.nx;
+            j3p.push_back(J);
+          } else {
+            if (!fits) lds3d = false;
+            J.block0 = nb3;
+            nb3 += (int64_t)J.gx * J.gy * s->g.nx;
+            j3.push_back(J);
+          }
         } else if (is_2z(s)) {
           if (J.m.nrec > kRecLds || J.m.nrec * chunks > 256) lds2z = false;
           J.bx = 256;
@@ -1969,6 +2015,7 @@ void build_jobs(gmes_batch* b, const std::vector<gmes_sim*>& sims, gmes_batch::J
       jg.push_back(G);
     }
     out.main3d[e] = {append_table(b->h_jobs, j3.data(), j3.size() * sizeof(MainJob<T>)), (int)j3.size(), nb3};
+    out.main3d_pp[e] = {append_table(b->h_jobs, j3p.data(), j3p.size() * sizeof(MainJob<T>)), (int)j3p.size(), nb3p};
     out.lds3d[e] = lds3d;
     out.lds2z[e] = lds2z;
     out.main2z[e] = {append_table(b->h_jobs, j2.data(), j2.size() * sizeof(MainJob<T>)), (int)j2.size(), nb2};
@@ -2037,6 +2084,8 @@ void enqueue_set_step_t(gmes_batch* b, const gmes_batch::JobSet& js, const std::
       btime_end(b, tm, st);
     }
     btime_begin(b, tm, e, st);
+    launch_main_batch<T>(electric, kBatch3dPP, job_ptr<MainJob<T>>(b, js.main3d_pp[e].off), js.main3d_pp[e].njobs,
+                         js.main3d_pp[e].nblocks, st);
     launch_main_batch<T>(electric, js.lds3d[e] ? kBatch3dLds : kBatch3d, job_ptr<MainJob<T>>(b, js.main3d[e].off),
                          js.main3d[e].njobs,
                          js.main3d[e].nblocks, st);
@@ -2205,7 +2254,8 @@ int gmes_batch_step(gmes_batch* b, int64_t nsteps) {
   std::vector<int64_t> sig;
   for (const gmes_batch::JobSet* js : {&b->set, &b->aux_set}) {
     for (int e = 0; e < 2; ++e)
-      for (const auto* L : {&js->main3d[e], &js->main2z[e], &js->mainnd[e], &js->src[e], &js->ghost[e]}) {
+      for (const auto* L : {&js->main3d[e], &js->main3d_pp[e], &js->main2z[e], &js->mainnd[e], &js->src[e],
+                            &js->ghost[e]}) {
         sig.push_back((int64_t)L->off);
         sig.push_back(L->njobs);
         sig.push_back(L->nblocks);

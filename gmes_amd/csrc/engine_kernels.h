@@ -71,6 +71,13 @@ struct MainArgs {
   unsigned cpml_mask;
   int prun[3];
   int po0[6][3];  // offsets of profile set 0's rows (component, axis) in the pool
+  // real-FP32 3-D kernels with CPML state in all three components of the
+  // field type: pp = 1 when, per axis, the two components that read a
+  // profile along it have identical set-0 rows; pprow[a] is then the pool
+  // offset of one of them, and the kernels load the row of each axis with
+  // the cell's other operands (lanes outside the zero-c run of that axis)
+  int pp;
+  int pprow[3];
   // real-FP32 3-D kernels: components whose UPML state word is loaded with
   // the other operands (bit c), and a box (per axis lo | hi << 16) that
   // holds no UPML cell of those components
@@ -210,7 +217,10 @@ constexpr int k2zRows = 1;
 // kinds of batched main launch
 // kBatch3dLds / kBatch2zLds: 3-D / collapsed-z jobs whose record tables
 // all fit the LDS staging (RecStage::use_lds for 256-thread blocks)
-constexpr int kBatchGeneric = 0, kBatch3d = 1, kBatch2z = 2, kBatch3dLds = 3, kBatch2zLds = 4;
+// kBatch3dPP: real-FP32 3-D jobs whose records fit the LDS staging and
+// whose CPML profile rows the kernel loads with the other operands
+// (MainArgs::pp)
+constexpr int kBatchGeneric = 0, kBatch3d = 1, kBatch2z = 2, kBatch3dLds = 3, kBatch2zLds = 4, kBatch3dPP = 5;
 template <class T>
 void launch_main_batch(bool electric, int kind, const MainJob<T>* jobs, int njobs, int64_t nblocks,
                        hipStream_t st);

@@ -185,10 +185,17 @@ __device__ __forceinline__ bool psi_live(const DevRec<R>& rec, const int ix[3]) 
 //   src/pw_cpml.hh:127-132,339-344, src/pw_upml.hh:124-128,328-332.
 //   FP32 uses reciprocals the host computed in FP64 (1/d, 1/kappa in the
 //   CPML profile, 1/eps in the record) and explicit FMAs.
-template <int C, class T, bool PRE>
+// CPML profile rows (b, c, 1/kappa) of the cell's coordinate along x, y, z,
+// loaded with the other operands (MainArgs::pp); c = 0, 1/kappa = 1 inside
+// the zero-c run of the axis.
+struct PRows {
+  float v[3][3];
+};
+
+template <int C, class T, bool PRE, bool PP = false>
 __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typename real_of<T>::type>& rec,
                                           T* __restrict__ st, int flat, int i, int j, int k, T f, T D1, T D2,
-                                          T pre1, T pre2) {
+                                          T pre1, T pre2, const PRows& pr) {
   using R = typename real_of<T>::type;
   constexpr bool E = C < 3;
   const int cls = rec.cls;
@@ -201,30 +208,39 @@ __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typ
     const GMES_GLOBAL int64_t* po = (const GMES_GLOBAL int64_t*)m.prof_off + (rec.prof * 6 + C) * 3;
     const bool set0 = rec.prof == 0;
     if (cls == GMES_CLS_CPML) {
-      const GMES_GLOBAL float* p1 = (const GMES_GLOBAL float*)m.prof + (set0 ? (int64_t)m.po0[C][Ax<C>::a1] : po[Ax<C>::a1]) + ix[Ax<C>::a1] * 3;
-      const GMES_GLOBAL float* p2 = (const GMES_GLOBAL float*)m.prof + (set0 ? (int64_t)m.po0[C][Ax<C>::a2] : po[Ax<C>::a2]) + ix[Ax<C>::a2] * 3;
+      float b1, c1, k1, b2, c2, k2;
+      if (PP && set0) {
+        b1 = pr.v[Ax<C>::a1][0]; c1 = pr.v[Ax<C>::a1][1]; k1 = pr.v[Ax<C>::a1][2];
+        b2 = pr.v[Ax<C>::a2][0]; c2 = pr.v[Ax<C>::a2][1]; k2 = pr.v[Ax<C>::a2][2];
+      } else {
+        const GMES_GLOBAL float* p1 = (const GMES_GLOBAL float*)m.prof + (set0 ? (int64_t)m.po0[C][Ax<C>::a1] : po[Ax<C>::a1]) + ix[Ax<C>::a1] * 3;
+        const GMES_GLOBAL float* p2 = (const GMES_GLOBAL float*)m.prof + (set0 ? (int64_t)m.po0[C][Ax<C>::a2] : po[Ax<C>::a2]) + ix[Ax<C>::a2] * 3;
+        b1 = p1[0]; c1 = p1[1]; k1 = p1[2];
+        b2 = p2[0]; c2 = p2[1]; k2 = p2[2];
+      }
       const bool u1 = psi_live<Ax<C>::a1>(rec, ix), u2 = psi_live<Ax<C>::a2>(rec, ix);
       T psi1, psi2;
       if constexpr (sizeof(T) >= 8) {
         const T o1 = u1 ? gld(st + flat) : vzero<T>();
         const T o2 = u2 ? gld(st + m.dsize[C] + flat) : vzero<T>();
-        psi1 = fma_(p1[1], a, vscl(p1[0], o1));
-        psi2 = fma_(p2[1], b, vscl(p2[0], o2));
+        psi1 = fma_(c1, a, vscl(b1, o1));
+        psi2 = fma_(c2, b, vscl(b2, o2));
         if (u1) st_nt(st + flat, psi1);
         if (u2) st_nt(st + m.dsize[C] + flat, psi2);
       } else {
-        // real FP32: branches around the psi words cost more than their
-        // traffic (measured).  A dead psi is read from element 0 of its
-        // array instead, a guard word that is always +0 (one cache line
-        // for all such lanes), and its +0 is stored back unconditionally.
-        const T o1 = PRE ? pre1 : gld(st + (u1 ? flat : 0));
-        const T o2 = PRE ? pre2 : gld(st + m.dsize[C] + (u2 ? flat : 0));
-        psi1 = fma_(p1[1], a, vscl(p1[0], o1));
-        psi2 = fma_(p2[1], b, vscl(p2[0], o2));
-        st_nt(st + flat, psi1);  // used once per launch, like the field words
-        st_nt(st + m.dsize[C] + flat, psi2);
+        // real FP32: a dead psi (coordinate inside the zero-c run of its
+        // axis) is neither loaded nor stored; it is +0 in memory and in the
+        // formula.  With the profile rows prefetched, masking these lanes
+        // beat reading a guard word and storing +0 back: 256^3 E launch
+        // 0.181 to 0.171 ms.
+        const T o1 = PRE ? pre1 : (u1 ? gld(st + flat) : T{});
+        const T o2 = PRE ? pre2 : (u2 ? gld(st + m.dsize[C] + flat) : T{});
+        psi1 = fma_(c1, a, vscl(b1, o1));
+        psi2 = fma_(c2, b, vscl(b2, o2));
+        if (u1) st_nt(st + flat, psi1);  // used once per launch, like the field words
+        if (u2) st_nt(st + m.dsize[C] + flat, psi2);
       }
-      const T t = vsub(vadd(fma_(-p2[2], b, vscl(p1[2], a)), psi1), psi2);
+      const T t = vsub(vadd(fma_(-k2, b, vscl(k1, a)), psi1), psi2);
       return E ? fma_(rec.c0, t, f) : fma_(-rec.c0, t, f);
     }
     // UPML
@@ -277,10 +293,10 @@ __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typ
 // Update of one cell of component C: Dummy (no-op), Const (src/pw_const.hh
 // :96,198), Dielectric, CPML, UPML.  ADE classes are left to their kernels.
 // f is the cell's old value, loaded by the caller.
-template <int C, class T, bool PRE = false>
+template <int C, class T, bool PRE = false, bool PP = false>
 __device__ __forceinline__ T store_cell(const MainArgs<T>& m, const DevRec<typename real_of<T>::type>& rec,
                                         T* __restrict__ F, T* __restrict__ st, int flat, int i, int j, int k,
-                                        T f, T D1, T D2, T pre1 = T{}, T pre2 = T{}) {
+                                        T f, T D1, T D2, T pre1 = T{}, T pre2 = T{}, const PRows& pr = PRows{}) {
   const int cls = rec.cls;
   if (cls == GMES_CLS_DUMMY || cls >= GMES_CLS_DRUDE) return f;
   if (cls == GMES_CLS_CONST) {
@@ -288,7 +304,7 @@ __device__ __forceinline__ T store_cell(const MainArgs<T>& m, const DevRec<typen
     st_nt(F + flat, v);
     return v;
   }
-  const T v = update_value<C, T, PRE>(m, rec, st, flat, i, j, k, f, D1, D2, pre1, pre2);
+  const T v = update_value<C, T, PRE, PP>(m, rec, st, flat, i, j, k, f, D1, D2, pre1, pre2, pr);
   st_nt(F + flat, v);
   return v;
 }
@@ -548,9 +564,9 @@ __global__ void __launch_bounds__(256) k_main2z(MainArgs<T> m) {
 // loaded with the cell's other operands, so a CPML cell needs no further
 // dependent load after its record.  Outside the coordinate runs where
 // every CPML profile's c is zero (m.prun) the word is live; inside, it is
-// +0 and is read from the array's always-zero guard word instead.  A
-// component with UPML state (and no CPML) gets its one state word the same
-// way, from the guard word inside m.ubox, a box that holds no UPML cell.
+// +0 and the lane loads nothing.  A component with UPML state (and no
+// CPML) gets its one state word the same way, loaded outside m.ubox, a box
+// that holds no UPML cell.
 // UP: the launch may carry UPML components (m.upml_mask); kernels without
 // it run the shorter CPML-only form (measured 1.2% faster at 256^3).
 template <int C, class T, bool UP>
@@ -563,8 +579,8 @@ __device__ __forceinline__ void prefetch_psi(const MainArgs<T>& m, int flat, con
     const int x1 = ix[Ax<C>::a1], x2 = ix[Ax<C>::a2];
     const bool u1 = x1 < (r1 & 0xFFFF) || x1 >= (r1 >> 16);
     const bool u2 = x2 < (r2 & 0xFFFF) || x2 >= (r2 >> 16);
-    o1 = gld(st + (u1 ? flat : 0));
-    o2 = gld(st + m.dsize[C] + (u2 ? flat : 0));
+    if (u1) o1 = gld(st + flat);
+    if (u2) o2 = gld(st + m.dsize[C] + flat);
     return;
   }
   const bool cp = (m.cpml_mask >> C) & 1u;
@@ -577,8 +593,8 @@ __device__ __forceinline__ void prefetch_psi(const MainArgs<T>& m, int flat, con
   for (int a = 0; a < 3; ++a) in = in && ix[a] >= (m.ubox[a] & 0xFFFF) && ix[a] < (m.ubox[a] >> 16);
   const bool u1 = cp ? x1 < (r1 & 0xFFFF) || x1 >= (r1 >> 16) : !in;
   const bool u2 = cp && (x2 < (r2 & 0xFFFF) || x2 >= (r2 >> 16));
-  o1 = gld(st + (u1 ? flat : 0));
-  o2 = gld(st + (cp ? m.dsize[C] : 0) + (u2 ? flat : 0));
+  if (u1) o1 = gld(st + flat);
+  if (u2) o2 = gld(st + m.dsize[C] + flat);
 }
 
 template <class T, int N>
@@ -587,9 +603,31 @@ struct Planes {
   T o[3][N];     // old values of the three updated components
   uint16_t r[3][N];
   T s[3][2][N];  // prefetched CPML psi words (real FP32)
+  PRows pr;      // prefetched CPML profile rows (real FP32, one plane, MainArgs::pp)
 };
 
-template <class T, int N, bool UP>
+// The CPML profile rows of coordinates x[0..2] (MainArgs::pp): a lane
+// outside the zero-c run of an axis loads that axis's row; inside it the
+// row is (b, 0, 1), and b multiplies a psi that is +0 there.
+template <class T>
+__device__ __forceinline__ void prefetch_rows(const MainArgs<T>& m, const int x[3], PRows& pr) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int r = m.prun[a];
+    if (x[a] < (r & 0xFFFF) || x[a] >= (r >> 16)) {
+      const GMES_GLOBAL float* p = (const GMES_GLOBAL float*)m.prof + m.pprow[a] + x[a] * 3;
+      pr.v[a][0] = p[0];
+      pr.v[a][1] = p[1];
+      pr.v[a][2] = p[2];
+    } else {
+      pr.v[a][0] = 0.f;
+      pr.v[a][1] = 0.f;
+      pr.v[a][2] = 1.f;
+    }
+  }
+}
+
+template <class T, int N, bool UP, bool PP = false>
 __device__ __forceinline__ void load_e(const MainArgs<T>& m, Planes<T, N>& q, int i0, int j, int k, const int* B, const int* P,
                                        const int* W, const T* __restrict__ hx, const T* __restrict__ hy,
                                        const T* __restrict__ hz, const T* __restrict__ ex, const T* __restrict__ ey,
@@ -626,13 +664,14 @@ __device__ __forceinline__ void load_e(const MainArgs<T>& m, Planes<T, N>& q, in
       prefetch_psi<0, T, UP>(m, fx, ix, q.s[0][0][n], q.s[0][1][n]);
       prefetch_psi<1, T, UP>(m, fy, ix, q.s[1][0][n], q.s[1][1][n]);
       prefetch_psi<2, T, UP>(m, fz, ix, q.s[2][0][n], q.s[2][1][n]);
+      if constexpr (PP) prefetch_rows<T>(m, ix, q.pr);  // E: x at i, y at j, z at k
     }
   }
 }
 
 // prev: H'z(i0-1,j,k) = Hz[i0, j+1, k] and H'y(i0-1,j,k) = Hy[i0, j, k+1],
 // the i-1 operands of the first plane; advanced to the last plane's.
-template <class T, int N>
+template <class T, int N, bool PP = false>
 __device__ __forceinline__ void update_e(const MainArgs<T>& m, const DevRec<typename real_of<T>::type>* recs,
                                          const Planes<T, N>& q, int i0, int j, int k, const int* B, const int* P,
                                          const int* W, T* __restrict__ ex, T* __restrict__ ey, T* __restrict__ ez,
@@ -644,15 +683,15 @@ __device__ __forceinline__ void update_e(const MainArgs<T>& m, const DevRec<type
     const int i = i0 + n;
     const int fx = B[0] + i * P[0] + j * W[0] + k, fy = B[1] + i * P[1] + j * W[1] + k;
     const int fz = B[2] + i * P[2] + j * W[2] + k;
-    const T vx = store_cell<0, T, kPre>(m, recs[q.r[0][n]], ex, sx, fx, i, j, k, q.o[0][n],
-                                        vsub(q.c[5][n], q.c[6][n]), vsub(q.c[3][n], q.c[4][n]),
-                                        q.s[0][0][n], q.s[0][1][n]);
-    const T vy = store_cell<1, T, kPre>(m, recs[q.r[1][n]], ey, sy, fy, i, j, k, q.o[1][n],
-                                        vsub(q.c[0][n], q.c[2][n]), vsub(q.c[5][n], hz_prev),
-                                        q.s[1][0][n], q.s[1][1][n]);
-    const T vz = store_cell<2, T, kPre>(m, recs[q.r[2][n]], ez, sz, fz, i, j, k, q.o[2][n],
-                                        vsub(q.c[3][n], hy_prev), vsub(q.c[0][n], q.c[1][n]),
-                                        q.s[2][0][n], q.s[2][1][n]);
+    const T vx = store_cell<0, T, kPre, PP>(m, recs[q.r[0][n]], ex, sx, fx, i, j, k, q.o[0][n],
+                                            vsub(q.c[5][n], q.c[6][n]), vsub(q.c[3][n], q.c[4][n]),
+                                            q.s[0][0][n], q.s[0][1][n], q.pr);
+    const T vy = store_cell<1, T, kPre, PP>(m, recs[q.r[1][n]], ey, sy, fy, i, j, k, q.o[1][n],
+                                            vsub(q.c[0][n], q.c[2][n]), vsub(q.c[5][n], hz_prev),
+                                            q.s[1][0][n], q.s[1][1][n], q.pr);
+    const T vz = store_cell<2, T, kPre, PP>(m, recs[q.r[2][n]], ez, sz, fz, i, j, k, q.o[2][n],
+                                            vsub(q.c[3][n], hy_prev), vsub(q.c[0][n], q.c[1][n]),
+                                            q.s[2][0][n], q.s[2][1][n], q.pr);
     if (m.fuse && (i == 0 || j == 0 || k == 0)) {
       const int ix[3] = {i, j, k};
       fused_ghost<T>(m, 0, ex, ix, vx);
@@ -664,7 +703,7 @@ __device__ __forceinline__ void update_e(const MainArgs<T>& m, const DevRec<type
   }
 }
 
-template <class T, int N, bool UP>
+template <class T, int N, bool UP, bool PP = false>
 __device__ __forceinline__ void load_h(const MainArgs<T>& m, Planes<T, N>& q, int i0, int j, int k, const int* B, const int* P,
                                        const int* W, const T* __restrict__ ex, const T* __restrict__ ey,
                                        const T* __restrict__ ez, const T* __restrict__ hx, const T* __restrict__ hy,
@@ -702,12 +741,16 @@ __device__ __forceinline__ void load_h(const MainArgs<T>& m, Planes<T, N>& q, in
       prefetch_psi<3, T, UP>(m, fx, ixx, q.s[0][0][n], q.s[0][1][n]);
       prefetch_psi<4, T, UP>(m, fy, ixy, q.s[1][0][n], q.s[1][1][n]);
       prefetch_psi<5, T, UP>(m, fz, ixz, q.s[2][0][n], q.s[2][1][n]);
+      if constexpr (PP) {
+        const int x[3] = {i + 1, j + 1, k + 1};  // H: x at i+1, y at j+1, z at k+1
+        prefetch_rows<T>(m, x, q.pr);
+      }
     }
   }
 }
 
 // cur: Ey(i0,j,k), Ez(i0,j,k), advanced to the last plane's i+1 values.
-template <class T, int N>
+template <class T, int N, bool PP = false>
 __device__ __forceinline__ void update_h(const MainArgs<T>& m, const DevRec<typename real_of<T>::type>* recs,
                                          const Planes<T, N>& q, int i0, int j, int k, const int* B, const int* P,
                                          const int* W, T* __restrict__ hx, T* __restrict__ hy, T* __restrict__ hz,
@@ -720,14 +763,15 @@ __device__ __forceinline__ void update_h(const MainArgs<T>& m, const DevRec<type
     const int fx = B[3] + i * P[3] + (j + 1) * W[3] + k + 1;
     const int fy = B[4] + (i + 1) * P[4] + j * W[4] + k + 1;
     const int fz = B[5] + (i + 1) * P[5] + (j + 1) * W[5] + k;
-    const T vx = store_cell<3, T, kPre>(m, recs[q.r[0][n]], hx, sx, fx, i, j + 1, k + 1, q.o[0][n],
-                                        vsub(q.c[5][n], ez_cur), vsub(q.c[3][n], ey_cur), q.s[0][0][n], q.s[0][1][n]);
-    const T vy = store_cell<4, T, kPre>(m, recs[q.r[1][n]], hy, sy, fy, i + 1, j, k + 1, q.o[1][n],
-                                        vsub(q.c[1][n], q.c[0][n]), vsub(q.c[6][n], ez_cur),
-                                        q.s[1][0][n], q.s[1][1][n]);
-    const T vz = store_cell<5, T, kPre>(m, recs[q.r[2][n]], hz, sz, fz, i + 1, j + 1, k, q.o[2][n],
-                                        vsub(q.c[4][n], ey_cur), vsub(q.c[2][n], q.c[0][n]),
-                                        q.s[2][0][n], q.s[2][1][n]);
+    const T vx = store_cell<3, T, kPre, PP>(m, recs[q.r[0][n]], hx, sx, fx, i, j + 1, k + 1, q.o[0][n],
+                                            vsub(q.c[5][n], ez_cur), vsub(q.c[3][n], ey_cur), q.s[0][0][n],
+                                            q.s[0][1][n], q.pr);
+    const T vy = store_cell<4, T, kPre, PP>(m, recs[q.r[1][n]], hy, sy, fy, i + 1, j, k + 1, q.o[1][n],
+                                            vsub(q.c[1][n], q.c[0][n]), vsub(q.c[6][n], ez_cur),
+                                            q.s[1][0][n], q.s[1][1][n], q.pr);
+    const T vz = store_cell<5, T, kPre, PP>(m, recs[q.r[2][n]], hz, sz, fz, i + 1, j + 1, k, q.o[2][n],
+                                            vsub(q.c[4][n], ey_cur), vsub(q.c[2][n], q.c[0][n]),
+                                            q.s[2][0][n], q.s[2][1][n], q.pr);
     ey_cur = q.c[4][n];
     ez_cur = q.c[6][n];
   }
@@ -743,9 +787,10 @@ __device__ __forceinline__ void update_h(const MainArgs<T>& m, const DevRec<type
 // LDSO: the launch guarantees that every block stages its records in LDS
 // (RecStage::use_lds), so one copy reads them from LDS only (batched
 // kernel, 7% faster than the generic pointer).  UP: see prefetch_psi.
-template <bool E, class T, int N, bool SPLIT, bool UP, bool LDSO = false>
+template <bool E, class T, int N, bool SPLIT, bool UP, bool LDSO = false, bool PP = false>
 __device__ __forceinline__ void main3d_tile(const MainArgs<T>& m, DevRec<typename real_of<T>::type>* srec, int k,
                                             int j, int i0, int tid, int nthreads) {
+  static_assert(!PP || (N == 1 && LDSO && std::is_same<T, float>::value), "row prefetch: real FP32, one plane, LDS records");
   using R = typename real_of<T>::type;
   const int nx = m.n[0], ny = m.n[1], nz = m.n[2];
   const bool act = k < nz && j < ny;  // every thread reaches the barrier
@@ -771,11 +816,11 @@ __device__ __forceinline__ void main3d_tile(const MainArgs<T>& m, DevRec<typenam
     T u0 = T{}, u1 = T{};
     if (act) {
       if constexpr (E) {
-        load_e<T, N, UP>(m, q, i0, j, k, B, P, W, f[3], f[4], f[5], f[0], f[1], f[2], ida, idb, idc, i8a, i8b, i8c, m.narrow != 0);
+        load_e<T, N, UP, PP>(m, q, i0, j, k, B, P, W, f[3], f[4], f[5], f[0], f[1], f[2], ida, idb, idc, i8a, i8b, i8c, m.narrow != 0);
         u0 = gld(f[5] + B[5] + i0 * P[5] + (j + 1) * W[5] + k);
         u1 = gld(f[4] + B[4] + i0 * P[4] + j * W[4] + k + 1);
       } else {
-        load_h<T, N, UP>(m, q, i0, j, k, B, P, W, f[0], f[1], f[2], f[3], f[4], f[5], ida, idb, idc, i8a, i8b, i8c, m.narrow != 0);
+        load_h<T, N, UP, PP>(m, q, i0, j, k, B, P, W, f[0], f[1], f[2], f[3], f[4], f[5], ida, idb, idc, i8a, i8b, i8c, m.narrow != 0);
         u0 = gld(f[1] + B[1] + i0 * P[1] + j * W[1] + k);
         u1 = gld(f[2] + B[2] + i0 * P[2] + j * W[2] + k);
       }
@@ -783,8 +828,8 @@ __device__ __forceinline__ void main3d_tile(const MainArgs<T>& m, DevRec<typenam
     stage.commit(srec, m.nrec);
     if (!act) return;
     if constexpr (LDSO) {
-      if constexpr (E) update_e<T, N>(m, srec, q, i0, j, k, B, P, W, f[0], f[1], f[2], sx, sy, sz, u0, u1);
-      else update_h<T, N>(m, srec, q, i0, j, k, B, P, W, f[3], f[4], f[5], sx, sy, sz, u0, u1);
+      if constexpr (E) update_e<T, N, PP>(m, srec, q, i0, j, k, B, P, W, f[0], f[1], f[2], sx, sy, sz, u0, u1);
+      else update_h<T, N, PP>(m, srec, q, i0, j, k, B, P, W, f[3], f[4], f[5], sx, sy, sz, u0, u1);
     } else if constexpr (!SPLIT) {
       const DevRec<R>* recs = stage.use_lds ? srec : grec;
       if constexpr (E) update_e<T, N>(m, recs, q, i0, j, k, B, P, W, f[0], f[1], f[2], sx, sy, sz, u0, u1);
@@ -837,11 +882,11 @@ template <> struct MainWaves<double, 1, true> { static constexpr int v = 7; };
 // LDSO: the launch guarantees the record table fits the LDS staging (see
 // main3d_tile); otherwise the update code is split into an LDS and a
 // global copy.
-template <bool E, class T, int N, bool UP, bool LDSO>
+template <bool E, class T, int N, bool UP, bool LDSO, bool PP = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MainWaves<T, N, LDSO>::v)))
 k_main3d(MainArgs<T> m) {
   __shared__ DevRec<typename real_of<T>::type> srec[kRecLds];
-  main3d_tile<E, T, N, !LDSO, UP, LDSO>(m, srec, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
+  main3d_tile<E, T, N, !LDSO, UP, LDSO, PP>(m, srec, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
                        blockIdx.z * N, threadIdx.y * blockDim.x + threadIdx.x, blockDim.x * blockDim.y);
 }
 
@@ -852,7 +897,7 @@ k_main3d(MainArgs<T> m) {
 // (uniform across the block) and runs the same tile body as the single-sim
 // kernel, with the member's own tile shape.
 
-template <bool E, class T, bool LDSO>
+template <bool E, class T, bool LDSO, bool PP = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MainWaves<T, 1>::v)))
 k_main3d_batch(const MainJob<T>* jobs, int njobs) {
   __shared__ DevRec<typename real_of<T>::type> srec[kRecLds];
@@ -861,7 +906,7 @@ k_main3d_batch(const MainJob<T>* jobs, int njobs) {
   const int bxi = local % J.gx, byi = (local / J.gx) % J.gy, bzi = local / (J.gx * J.gy);
   const int tid = threadIdx.x;
   const int tx = tid & (J.bx - 1), ty = tid / J.bx;
-  main3d_tile<E, T, 1, false, true, LDSO>(J.m, srec, bxi * J.bx + tx, byi * (256 / J.bx) + ty, bzi, tid, 256);
+  main3d_tile<E, T, 1, false, true, LDSO, PP>(J.m, srec, bxi * J.bx + tx, byi * (256 / J.bx) + ty, bzi, tid, 256);
 }
 
 template <bool E, class T>
@@ -1188,18 +1233,29 @@ __global__ void __launch_bounds__(256) k_dft_batch(const DftJob<T>* jobs, int nj
   const double vr = (double)re_of(v), vi = (double)im_of(v);
   // the job table's pointers are global memory: address them as such
   const GMES_GLOBAL double* w = (const GMES_GLOBAL double*)J.w + row * J.nfreq * 4 + (J.electric ? 0 : 2);
-  // each (re, im) accumulator pair is one 16-byte word, read and written
-  // once per sampled step: non-temporal
+  // each (re, im) accumulator pair and its (re, im) compensation pair are
+  // 16-byte words, read and written once per sampled step: non-temporal.
+  // Compensated (Kahan) summation: c carries the low-order part that the
+  // running sum lost, so the error does not grow with the number of steps.
   typedef double d2 __attribute__((ext_vector_type(2)));
   GMES_GLOBAL d2* acc = (GMES_GLOBAL d2*)J.acc + q;
+  GMES_GLOBAL d2* cmp = (GMES_GLOBAL d2*)J.acc + (int64_t)J.nfreq * J.ncells + q;
 #pragma unroll 4
   for (int f = 0; f < J.nfreq; ++f) {
     const double wr = w[4 * f], wi = w[4 * f + 1];
     GMES_GLOBAL d2* a = acc + f * J.ncells;
+    GMES_GLOBAL d2* e = cmp + f * J.ncells;
     d2 x = __builtin_nontemporal_load(a);
-    x.x = x.x + (wr * vr - wi * vi);
-    x.y = x.y + (wr * vi + wi * vr);
+    d2 c = __builtin_nontemporal_load(e);
+    const double yr = (wr * vr - wi * vi) - c.x;
+    const double yi = (wr * vi + wi * vr) - c.y;
+    const double tr = x.x + yr, ti = x.y + yi;
+    c.x = (tr - x.x) - yr;
+    c.y = (ti - x.y) - yi;
+    x.x = tr;
+    x.y = ti;
     __builtin_nontemporal_store(x, a);
+    __builtin_nontemporal_store(c, e);
   }
 }
 
@@ -1372,6 +1428,12 @@ void launch_main3d(bool electric, const MainArgs<T>& m, int bx, int by, int ichu
       if (lds) GMES_L3(1, true, true)
       GMES_L3(1, true, false) GMES_L3(2, true, false) GMES_L3(4, true, false) GMES_L3(8, true, false)
     }
+    // CPML profile rows loaded with the other operands (MainArgs::pp)
+    if (m.pp && lds && n == 1) {
+      if (electric) hipLaunchKernelGGL((k_main3d<true, T, 1, false, true, true>), grid, block, 0, st, m);
+      else hipLaunchKernelGGL((k_main3d<false, T, 1, false, true, true>), grid, block, 0, st, m);
+      return;
+    }
   }
   if (lds) GMES_L3(1, false, true)
   GMES_L3(1, false, false) GMES_L3(2, false, false) GMES_L3(4, false, false) GMES_L3(8, false, false)
@@ -1399,7 +1461,12 @@ void launch_main_batch(bool electric, int kind, const MainJob<T>* jobs, int njob
                        hipStream_t st) {
   if (njobs == 0 || nblocks == 0) return;
   const dim3 grid((unsigned)nblocks), block(256);
-  if (kind == kBatch3dLds) {
+  if (kind == kBatch3dPP) {
+    if constexpr (std::is_same<T, float>::value) {
+      if (electric) hipLaunchKernelGGL((k_main3d_batch<true, T, true, true>), grid, block, 0, st, jobs, njobs);
+      else hipLaunchKernelGGL((k_main3d_batch<false, T, true, true>), grid, block, 0, st, jobs, njobs);
+    }
+  } else if (kind == kBatch3dLds) {
     if (electric) hipLaunchKernelGGL((k_main3d_batch<true, T, true>), grid, block, 0, st, jobs, njobs);
     else hipLaunchKernelGGL((k_main3d_batch<false, T, true>), grid, block, 0, st, jobs, njobs);
   } else if (kind == kBatch3d) {

@@ -163,6 +163,7 @@ class FDTD(object):
         self._lib = None
         self._dft_monitors = []
         self._dft_freqs = None
+        self._wave_win = self._dft_win = (0, 0)  # steps covered by the uploaded windows
 
     # ---- setup ----------------------------------------------------------
 
@@ -249,6 +250,7 @@ class FDTD(object):
         self._sim = lib.sim_create(desc)
         self._dft_monitors = []
         self._dft_freqs = None
+        self._wave_win = self._dft_win = (0, 0)  # steps covered by the uploaded windows
 
         if self.verbose:
             print("Mapping the piecewise material.")
@@ -289,6 +291,7 @@ class FDTD(object):
         cell in order of first appearance, and per cell its ops in order
         (the faces of a TFSF edge cell in face order)."""
         self._waves = []
+        self._wave_win = (0, 0)
         env_rows = {}
         ops = []
 
@@ -394,18 +397,30 @@ class FDTD(object):
 
     # ---- stepping -------------------------------------------------------
 
+    # steps of waveform samples and DFT weights uploaded at a time: a loop
+    # of step() calls then enqueues its steps without a host round trip
+    # (the upload waits for the device) except once per window
+    WINDOW = 256
+
     def _prepare(self, nsteps):
-        """Upload dirty mirrors and waveforms for the next nsteps (also for
-        the auxiliary grids)."""
+        """Upload dirty mirrors, and waveforms and DFT weights unless the
+        uploaded window already covers the next nsteps (also for the
+        auxiliary grids)."""
         self._push_dirty()
         step0 = self._lib.get_step(self._sim)
-        if self._waves:
-            vals = np.stack([w.sample(step0, nsteps, self.time_step.dt) for w in self._waves])
+        end = step0 + nsteps
+        if self._waves and not (self._wave_win[0] <= step0 and end <= self._wave_win[1]):
+            exact = any(w.exact for w in self._waves)
+            n = nsteps if exact else max(nsteps, self.WINDOW)
+            vals = np.stack([w.sample(step0, n, self.time_step.dt) for w in self._waves])
             self._lib.set_waveforms(self._sim, step0, vals)
-        if self._dft_monitors:
+            self._wave_win = (step0, step0 + n)
+        if self._dft_monitors and not (self._dft_win[0] <= step0 and end <= self._dft_win[1]):
             from .monitor import dft_weights
+            n = max(nsteps, self.WINDOW)
             self._lib.set_dft_weights(self._sim, step0,
-                                      dft_weights(self._dft_freqs, step0, nsteps, self.time_step.dt))
+                                      dft_weights(self._dft_freqs, step0, n, self.time_step.dt))
+            self._dft_win = (step0, step0 + n)
         for aux in self._aux_fdtds:
             if aux._sim is not None:  # a closed grid: the engine refuses the step
                 aux._prepare(nsteps)

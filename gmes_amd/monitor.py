@@ -45,6 +45,7 @@ class DftBox(object):
         self._id = sim._lib.add_dft(sim._sim, comp.index, self.lo, self.hi, len(self.freqs),
                                     self.decimation)
         sim._dft_monitors.append(self)
+        sim._dft_win = (0, 0)  # weights are uploaded again before the next step
 
     def values(self):
         n = int(np.prod(self.box))

@@ -134,12 +134,15 @@ def _record_source_value(output, time, value):
 
 
 class WaveRow(object):
-    """Host generator of one waveform row: value(n) for the half step n."""
+    """Host generator of one waveform row: value(n) for the half step n.
+    exact: the row has side effects per evaluation (a source that records
+    its values), so it is sampled for the steps about to run only, never
+    ahead."""
 
-    def __init__(self, fn, electric, recorder=None):
+    def __init__(self, fn, electric, exact=False):
         self.fn = fn
         self.electric = electric
-        self.recorder = recorder
+        self.exact = exact
 
     def sample(self, step0, nsteps, dt):
         out = np.empty(nsteps, np.complex128)
@@ -204,7 +207,7 @@ class PointSource(Src):
             return dt * src_t / material
 
         kind = SRC_SET if hard else SRC_SUB
-        return [(tuple(int(v) for v in idx), "point", kind, WaveRow(value, electric))]
+        return [(tuple(int(v) for v in idx), "point", kind, WaveRow(value, electric, exact=rec is not None))]
 
 
 # ---------------------------------------------------------------------------

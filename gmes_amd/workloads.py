@@ -220,13 +220,13 @@ def serve_requests(n=48, rate=20.0, seed=5, lo=64, hi=256):
 def pc_slab_footprint(side, nfreq=50, complex_fp32=True):
     """Device bytes of a served slab solve, estimated before it is built:
     six fields, six id arrays, full-grid CPML state of the six components,
-    and the four DFT boxes of the flux plane."""
+    and the four DFT boxes of the flux plane (sums and compensations)."""
     es = 8 if complex_fp32 else 16
     cells = float(side) ** 3
     fields = cells * 6 * es
     ids = cells * 6 * 2
     cpml = cells * 6 * 2 * es
-    dft = nfreq * 6 * float(side) ** 2 * 16
+    dft = nfreq * 6 * float(side) ** 2 * 32  # compensated sums: two (re, im) pairs per cell
     return int(1.05 * (fields + ids + cpml + dft)) + (64 << 20)
 
 

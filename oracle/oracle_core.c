@@ -326,7 +326,7 @@ static int64_t oblob_bytes(const gmes_sim *s) {
   int64_t b = 16;
   for (int c = 0; c < 6; ++c)
     for (int k = 0; k < GMES_NCLS; ++k) b += s->cls[c][k].n * s->cls[c][k].nstate * 16;
-  for (int d = 0; d < s->ndft; ++d) b += (int64_t)s->dft_nfreq * s->dft[d].ncells * 16;
+  for (int d = 0; d < s->ndft; ++d) b += (int64_t)s->dft_nfreq * s->dft[d].ncells * 32;
   for (int i = 0; i < s->naux; ++i) {
     const gmes_sim *a = s->aux[i];
     if (!a) continue;
@@ -351,7 +351,7 @@ static int64_t oblob_copy(gmes_sim *s, char *p, int save) {
       done += (int64_t)b;
     }
   for (int d = 0; d < s->ndft; ++d) {
-    size_t b = (size_t)((int64_t)s->dft_nfreq * s->dft[d].ncells * 16);
+    size_t b = (size_t)((int64_t)s->dft_nfreq * s->dft[d].ncells * 32);
     if (save) ocopy(p + done, s->dft[d].acc, b); else ocopy(s->dft[d].acc, p + done, b);
     done += (int64_t)b;
   }
@@ -472,6 +472,14 @@ static void apply_sources(gmes_sim *s, int c, int64_t step) {
 
 /* Running DFT at the end of a step: E values are those of n = k + 1/2 and
  * H values those of n = k + 1, each with its own weight of the row. */
+/* Compensated (Kahan) summation: *c keeps what the running sum *s lost. */
+static void kahan_add(double *s, double *c, double x) {
+  const double y = x - *c;
+  const double t = *s + y;
+  *c = (t - *s) - y;
+  *s = t;
+}
+
 static void accumulate_dft(gmes_sim *s) {
   if (s->ndft == 0) return;
   const int64_t row = s->step - s->dft_step0;
@@ -489,9 +497,10 @@ static void accumulate_dft(gmes_sim *s) {
       const ocv v = ocore_get(s, c, i + s->dft[d].lo[0], j + s->dft[d].lo[1], k + s->dft[d].lo[2]);
       for (int f = 0; f < s->dft_nfreq; ++f) {
         double *a = s->dft[d].acc + 2 * (f * s->dft[d].ncells + q);
+        double *e = a + 2 * (int64_t)s->dft_nfreq * s->dft[d].ncells;  /* compensations */
         const double wr = w[4 * f], wi = w[4 * f + 1];
-        a[0] = a[0] + (wr * v.re - wi * v.im);
-        a[1] = a[1] + (wr * v.im + wi * v.re);
+        kahan_add(&a[0], &e[0], wr * v.re - wi * v.im);
+        kahan_add(&a[1], &e[1], wr * v.im + wi * v.re);
       }
     }
   }
@@ -560,7 +569,7 @@ int gmes_sim_add_dft(gmes_sim *s, int32_t c, const int64_t lo[3], const int64_t 
     s->dft[d].ncells *= s->dft[d].n[a];
   }
   s->dft[d].decim = decim;
-  s->dft[d].acc = (double *)calloc((size_t)(nfreq * s->dft[d].ncells * 2), sizeof(double));
+  s->dft[d].acc = (double *)calloc((size_t)(2 * nfreq * s->dft[d].ncells * 2), sizeof(double));
   s->dft_nfreq = nfreq;
   s->ndft++;
   *id = d;

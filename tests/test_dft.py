@@ -116,3 +116,49 @@ def test_hard_source_cell_closed_form_gpu(engine_lib, comp, mode, half):
     sim._advance(60)
     got = box.values()[:, 0, 0, 0]
     assert np.allclose(got, closed_form(st, freqs, 60, sim.time_step.dt, half), rtol=1e-12)
+
+
+def _compensated_case(lib, steps):
+    """A hard-sourced Ez cell over many steps.  Compensated (Kahan)
+    summation keeps each accumulator within 2 eps sum|x_i| of the exact
+    sum of the FP64 terms w * F (math.fsum), whatever the number of steps,
+    and within 2 ulp where the terms do not cancel (the real part at the
+    source frequency);
+    an uncompensated running sum drifts there by thousands of ulp over
+    20000 steps."""
+    import math
+    from gmes_amd.monitor import dft_weights
+    sim, st = hard_source_case(lib, g.Ez, g.TMzFDTD)
+    idx = sim.space.space_to_index(g.Ez, 0.1, -0.2, 0)
+    freqs = (0.3, 0.7, 1.1)
+    box = DftBox(sim, g.Ez, idx, tuple(i + 1 for i in idx), freqs)
+    sim._advance(steps)
+    got = box.values()[:, 0, 0, 0]
+    dt = sim.time_step.dt
+    w = dft_weights(freqs, 0, steps, dt)
+    v = np.array([st.oscillator((k + 0.5) * dt) for k in range(steps)])
+    sim.close()
+    eps = np.finfo(float).eps
+    for fi, f in enumerate(freqs):
+        for part, terms in ((got[fi].real, [float(a) * float(b) - float(c) * 0.0
+                                            for a, b, c in zip(w[:, fi, 0, 0], v, w[:, fi, 0, 1])]),
+                            (got[fi].imag, [float(a) * 0.0 + float(c) * float(b)
+                                            for a, b, c in zip(w[:, fi, 0, 0], v, w[:, fi, 0, 1])])):
+            exact = math.fsum(terms)
+            mag = math.fsum(abs(x) for x in terms)
+            err = abs(part - exact)
+            assert err <= 2 * eps * mag, (f, err)
+            if abs(exact) >= 0.5 * mag:  # little cancellation (the source frequency)
+                assert err <= 2 * np.spacing(abs(exact)), (f, err)
+    return got
+
+
+def test_dft_accumulation_is_compensated_oracle(oracle_lib):
+    _compensated_case(oracle_lib, 20000)
+
+
+@pytest.mark.gpu
+def test_dft_accumulation_is_compensated_gpu(engine_lib, oracle_lib):
+    got = _compensated_case(engine_lib, 20000)
+    ref = _compensated_case(oracle_lib, 20000)
+    np.testing.assert_array_equal(got, ref)  # same terms, same order, same compensation

@@ -340,3 +340,23 @@ def test_fp32_3d_with_more_than_256_records(engine_lib, oracle_lib):
     err = rel_l2(run(many_records3d, engine_lib, steps, precision="fp32"), gpu)
     print("many records, FP32 vs FP64: rel L2 %.3g" % err)
     assert err <= 1e-5
+
+
+@pytest.mark.parametrize("bound", [g.Cpml, g.Upml], ids=["cpml", "upml"])
+def test_fp32_tile_shapes_give_the_same_bits(engine_lib, bound):
+    """Real FP32, 3-D: the default 128 x 2 tile stages the records in LDS
+    and, with CPML on every face, loads the profile rows with the other
+    operands; a 16 x 1 tile is too small to stage the 9 records (27
+    16-byte chunks) and reads them, and the rows, after each cell's
+    record.  Same arithmetic per cell: the fields must agree bit for bit."""
+    out = []
+    for tile in (None, (16, 1)):
+        sim = spheres3d(engine_lib, precision="fp32", n=40, bound=bound)
+        if tile:
+            engine_lib.set_tile(sim._sim, *tile)
+        sim._advance(80)
+        out.append(fields(sim))
+        sim.close()
+    for c in out[0]:
+        np.testing.assert_array_equal(out[0][c], out[1][c], err_msg=c.__name__)
+    assert np.abs(out[0][g.Ex]).max() > 0
