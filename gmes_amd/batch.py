@@ -5,8 +5,10 @@ A ``Batch`` groups initialised FDTD objects of one precision and field type
 and clock; a step of the batch runs the main, source and ghost kernels of
 every member as one launch each (the engine maps blocks to members through
 a device job table) and is replayed from one captured HIP graph.  Members
-may differ in size, mode and materials.  Before batching, each member's
-main-kernel tile is chosen by the engine's autotuner for its size class.
+may differ in size, mode and materials.  The batched main launches of 3-D
+and collapsed-z members run flat blocks of 256 consecutive cells, so they
+need no tile; autotune=True still tunes each 3-D member's tile for its own
+launches (when it is stepped alone).
 
 This has no counterpart in the reference, which advances one FDTD at a
 time (gmes/fdtd.py:1099-1122); every member computes exactly what it would
@@ -15,7 +17,7 @@ compute alone.
 
 
 class Batch(object):
-    def __init__(self, sims, autotune=True, graph=True):
+    def __init__(self, sims, autotune=False, graph=True):
         if not sims:
             raise ValueError("empty batch")
         self.sims = list(sims)

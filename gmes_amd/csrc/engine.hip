@@ -1971,8 +1971,8 @@ void build_jobs(gmes_batch* b, const std::vector<gmes_sim*>& sims, gmes_batch::J
         const int tby = 256 / bx;
         if (is3d(s)) {
           const bool fits = J.m.nrec <= kRecLds && J.m.nrec * chunks <= 256;
-          J.gx = (s->g.nz + bx - 1) / bx;
-          J.gy = (s->g.ny + tby - 1) / tby;
+          J.gx = (int)(((int64_t)s->g.ny * s->g.nz + 255) / 256);
+          J.gy = 1;
           if (std::is_same<T, float>::value && J.m.pp && fits) {
             J.block0 = nb3p;
             nb3p += (int64_t)J.gx * J.gy * s->g.nx;
@@ -1985,9 +1985,9 @@ void build_jobs(gmes_batch* b, const std::vector<gmes_sim*>& sims, gmes_batch::J
           }
         } else if (is_2z(s)) {
           if (J.m.nrec > kRecLds || J.m.nrec * chunks > 256) lds2z = false;
-          J.bx = 256;
-          J.gx = (ext[1] + 255) / 256;
-          J.gy = (ext[0] + k2zRows - 1) / k2zRows;
+          J.bx = ext[1];
+          J.gx = (int)(((int64_t)ext[0] * ext[1] + 255) / 256);
+          J.gy = 1;
           J.block0 = nb2;
           nb2 += (int64_t)J.gx * J.gy;
           j2.push_back(J);

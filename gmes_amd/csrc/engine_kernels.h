@@ -147,11 +147,17 @@ struct GhostArgs {
 
 // Batched-launch job tables: one entry per member sim; block0 is the
 // member's first block in the launch (entries sorted by block0).
+// 3-D and collapsed-z members run flat blocks of 256 consecutive points,
+// so rows that are not a multiple of a tile width leave no lanes idle:
+// 3-D, gx blocks per i plane over its row-major (j, k) points; collapsed
+// z, gx blocks over the row-major (i, j) plane whose rows are bx long.
+// Other members run bx x (256/bx) tiles, gx along k and gy along j per
+// plane.
 template <class T>
 struct MainJob {
   MainArgs<T> m;
-  int bx;      // tile width along k (power of two); the tile is bx x (256/bx)
-  int gx, gy;  // blocks along k and j; the rest of the member's blocks run along i
+  int bx;
+  int gx, gy;
   int pad_;
   int64_t block0;
 };

@@ -902,11 +902,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MainWa
 k_main3d_batch(const MainJob<T>* jobs, int njobs) {
   __shared__ DevRec<typename real_of<T>::type> srec[kRecLds];
   const MainJob<T>& J = jobs[find_job(jobs, njobs, blockIdx.x)];
-  const int local = (int)(blockIdx.x - J.block0);
-  const int bxi = local % J.gx, byi = (local / J.gx) % J.gy, bzi = local / (J.gx * J.gy);
-  const int tid = threadIdx.x;
-  const int tx = tid & (J.bx - 1), ty = tid / J.bx;
-  main3d_tile<E, T, 1, false, true, LDSO, PP>(J.m, srec, bxi * J.bx + tx, byi * (256 / J.bx) + ty, bzi, tid, 256);
+  const unsigned local = (unsigned)(blockIdx.x - J.block0);
+  const unsigned i = local / (unsigned)J.gx;
+  const unsigned f = (local - i * (unsigned)J.gx) * 256u + threadIdx.x;  // row-major (j, k) of plane i
+  const unsigned j = f / (unsigned)J.m.n[2];
+  // PP members have CPML state in every component of the field type, so no
+  // UPML state word to prefetch (UP = false, the shorter form)
+  main3d_tile<E, T, 1, false, !PP, LDSO, PP>(J.m, srec, (int)(f - j * (unsigned)J.m.n[2]), (int)j, (int)i,
+                                              threadIdx.x, 256);
 }
 
 template <bool E, class T>
@@ -924,15 +927,15 @@ __global__ void __launch_bounds__(256) k_main_batch(const MainJob<T>* jobs, int 
   }
 }
 
-// Collapsed-z 2-D members: a job's blocks are gx blocks of 256 j by gy
-// groups of k2zRows i rows.
+// Collapsed-z 2-D members: a job's gx blocks each take 256 consecutive
+// (i, j) columns of the row-major i x bx plane (bx: the extent along j).
 template <bool E, class T, bool LDSO>
 __global__ void __launch_bounds__(256) k_main2z_batch(const MainJob<T>* jobs, int njobs) {
   __shared__ DevRec<typename real_of<T>::type> srec[kRecLds];
   const MainJob<T>& J = jobs[find_job(jobs, njobs, blockIdx.x)];
-  const int local = (int)(blockIdx.x - J.block0);
-  const int bxi = local % J.gx, byi = local / J.gx;
-  main2z_tile<E, T, k2zRows, LDSO>(J.m, srec, byi * k2zRows, bxi * 256 + (int)threadIdx.x, threadIdx.x, 256);
+  const unsigned f = (unsigned)(blockIdx.x - J.block0) * 256u + threadIdx.x;
+  const unsigned i = f / (unsigned)J.bx;
+  main2z_tile<E, T, 1, LDSO>(J.m, srec, (int)i, (int)(f - i * (unsigned)J.bx), threadIdx.x, 256);
 }
 
 // Batched launches: block -> job by binary search on the jobs' first

@@ -63,7 +63,7 @@ class Request(object):
 
 
 class Server(object):
-    def __init__(self, hbm_budget, max_active=32, chunk=250, workers=4, autotune=True, clock=None,
+    def __init__(self, hbm_budget, max_active=32, chunk=250, workers=4, autotune=False, clock=None,
                  policy="arrival"):
         if policy not in ("arrival", "shortest"):
             raise ValueError("policy must be 'arrival' or 'shortest'")

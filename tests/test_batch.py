@@ -71,7 +71,7 @@ def test_oracle_batch_matches_single_steps(oracle_lib):
 @pytest.mark.parametrize("graph", [True, False], ids=["graph", "direct"])
 def test_batch_members_match_alone(engine_lib, precision, graph):
     alone = run_alone(real_members(engine_lib, precision), CHUNKS)
-    batched, tiles = run_batched(real_members(engine_lib, precision), CHUNKS, graph=graph)
+    batched, tiles = run_batched(real_members(engine_lib, precision), CHUNKS, graph=graph, autotune=True)
     assert all(t is not None for t in tiles)
     for i, (a, b) in enumerate(zip(alone, batched)):
         for c in a:
