@@ -459,6 +459,26 @@ def served_leg(nreq, budget_gb=64.0, max_active=16, chunk=250, policy="shortest"
                      "stream GB/s counts the main kernels' algorithmic bytes over the whole elapsed time")
 
 
+def copy_gbps(reps=20):
+    """Read-plus-write rate of torch's device copy of 256 MiB into 256 MiB
+    (HIP events): the rate a plain stream reaches on this part, beside the
+    8 TB/s nominal peak."""
+    import torch
+    src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return 2 * (64 << 20) * 4 / (ms * 1e-3) / 1e9
+
+
 def copy_probe():
     """A plain device-to-device copy of 256 MiB in, 256 MiB out, five times."""
     import torch
@@ -542,6 +562,11 @@ def main():
                     kernel="k_main3d<%s, float, 1, false, true>" % ("true" if fam == 0 else "false"),
                     bytes_per_launch=int(nbytes), avg_launch_ms=round(avg_ms, 5),
                     other_kernel_gbps=round(other_gbps, 1))
+    cg = copy_gbps()
+    roofline["copy_gbps"] = round(cg, 1)
+    if roofline["traffic"]:
+        roofline["traffic_gbps"] = round(roofline["traffic"] / (avg_ms * 1e-3) / 1e9, 1)
+    log("device copy (256 MiB): %.0f GB/s; kernel algorithmic %.0f GB/s" % (cg, achieved))
     fp32_steps_done = args.warmup + args.steps + prof_steps
 
     # ---- FP64 (the reference's precision) --------------------------------

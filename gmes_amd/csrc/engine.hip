@@ -290,6 +290,32 @@ void upml_box(gmes_sim* s) {
   for (int a = 0; a < 3; ++a) s->ubox[a] = (int32_t)(lo[a] | (hi[a] << 16));
 }
 
+// Real-FP32 3-D sims whose three components of a field type all get their
+// UPML state prefetched (upml_pre): each axis is a curl axis of two of
+// them (E: x of Ey and Ez, y of Ex and Ez, z of Ex and Ey; H: x of Hy and
+// Hz, y of Hx and Hz, z of Hx and Hy), which take c1..c4 at the same
+// coordinates, and the own axis of the third, which takes c5, c6.  When
+// the two set-0 rows of each axis are identical, the kernels load that
+// row and the owner's row with the cell's other operands (MainArgs::ur)
+// instead of three rows per component after the cell's record.
+void upml_rows(gmes_sim* s) {
+  s->ur_ok[0] = s->ur_ok[1] = false;
+  if (!is3d(s) || s->g.dtype != GMES_F32 || s->nprof < 1) return;
+  for (int e = 0; e < 2; ++e) {
+    const int c0 = e ? 3 : 0;
+    bool ok = true;
+    for (int c = c0; c < c0 + 3 && ok; ++c) ok = (s->upml_pre >> c) & 1u;
+    const int pairs[3][2] = {{c0 + 1, c0 + 2}, {c0, c0 + 2}, {c0, c0 + 1}};
+    for (int a = 0; a < 3 && ok; ++a) {
+      const std::vector<double>& r0 = s->prof_host[0][pairs[a][0]][a];
+      const std::vector<double>& r1 = s->prof_host[0][pairs[a][1]][a];
+      ok = s->prof_w[0][pairs[a][0]][a] == 6 && s->prof_w[0][pairs[a][1]][a] == 6 &&
+           s->prof_w[0][c0 + a][a] == 6 && !r0.empty() && r0 == r1 && !s->prof_host[0][c0 + a][a].empty();
+    }
+    s->ur_ok[e] = ok;
+  }
+}
+
 template <class R>
 int upload_tables(gmes_sim* s) {
   std::vector<DevRec<R>> dr(s->recs.size());
@@ -494,6 +520,17 @@ bool main_args(const gmes_sim* s, bool electric, MainArgs<T>& m, int ext[3]) {
   }
   for (int c = 0; c < 6; ++c)
     for (int a = 0; a < 3; ++a) m.po0[c][a] = (int)s->prof_off0[c][a];
+  m.ur = 0;
+  if (s->ur_ok[electric ? 0 : 1]) {
+    // shared rows: x of the 2nd component, y and z of the 1st; own rows:
+    // axis a of component a
+    const int c0 = electric ? 0 : 3;
+    m.ur = 1;
+    m.urs[0] = m.po0[c0 + 1][0];
+    m.urs[1] = m.po0[c0][1];
+    m.urs[2] = m.po0[c0][2];
+    for (int a = 0; a < 3; ++a) m.uro[a] = m.po0[c0 + a][a];
+  }
   m.pp = 0;
   if (s->pp_ok[electric ? 0 : 1]) {
     // axis x: 2nd component (Ey / Hy), axis y and z: 1st component (Ex / Hx)
@@ -1401,6 +1438,7 @@ int gmes_sim_commit(gmes_sim* s) {
   }
   profile_prefetch(s);
   upml_box(s);
+  upml_rows(s);
   s->committed = true;
   destroy_graph(s);
   decide_fusion(s);
@@ -1923,6 +1961,7 @@ struct gmes_batch {
   struct JobSet {
     Launch main3d[2], main2z[2], mainnd[2], src[2], ghost[2];  // [0] E, [1] H
     Launch main3d_pp[2];  // 3-D jobs with the CPML row prefetch (kBatch3dPP)
+    Launch main3d_ur[2];  // 3-D jobs with the UPML row prefetch (kBatch3dUR)
     bool lds3d[2] = {false, false};  // every 3-D job stages its records in LDS
     bool lds2z[2] = {false, false};  // every collapsed-z job does
     Launch ade[GMES_NCLS];                                      // by dispersive class
@@ -1952,8 +1991,8 @@ template <class T>
 void build_jobs(gmes_batch* b, const std::vector<gmes_sim*>& sims, gmes_batch::JobSet& out) {
   for (int e = 0; e < 2; ++e) {
     const bool electric = e == 0;
-    std::vector<MainJob<T>> j3, j2, jn, j3p;
-    int64_t nb3 = 0, nb2 = 0, nbn = 0, nb3p = 0;
+    std::vector<MainJob<T>> j3, j2, jn, j3p, j3u;
+    int64_t nb3 = 0, nb2 = 0, nbn = 0, nb3p = 0, nb3u = 0;
     bool lds3d = true, lds2z = true;
     // the block-wide record staging of the main tiles (RecStage::use_lds)
     constexpr int chunks = (int)(sizeof(DevRec<typename real_of<T>::type>) / 16);
@@ -1977,6 +2016,10 @@ void build_jobs(gmes_batch* b, const std::vector<gmes_sim*>& sims, gmes_batch::J
             J.block0 = nb3p;
             nb3p += (int64_t)J.gx * J.gy * s->g.nx;
             j3p.push_back(J);
+          } else if (std::is_same<T, float>::value && J.m.ur && fits) {
+            J.block0 = nb3u;
+            nb3u += (int64_t)J.gx * J.gy * s->g.nx;
+            j3u.push_back(J);
           } else {
             if (!fits) lds3d = false;
             J.block0 = nb3;
@@ -2016,6 +2059,7 @@ void build_jobs(gmes_batch* b, const std::vector<gmes_sim*>& sims, gmes_batch::J
     }
     out.main3d[e] = {append_table(b->h_jobs, j3.data(), j3.size() * sizeof(MainJob<T>)), (int)j3.size(), nb3};
     out.main3d_pp[e] = {append_table(b->h_jobs, j3p.data(), j3p.size() * sizeof(MainJob<T>)), (int)j3p.size(), nb3p};
+    out.main3d_ur[e] = {append_table(b->h_jobs, j3u.data(), j3u.size() * sizeof(MainJob<T>)), (int)j3u.size(), nb3u};
     out.lds3d[e] = lds3d;
     out.lds2z[e] = lds2z;
     out.main2z[e] = {append_table(b->h_jobs, j2.data(), j2.size() * sizeof(MainJob<T>)), (int)j2.size(), nb2};
@@ -2086,6 +2130,8 @@ void enqueue_set_step_t(gmes_batch* b, const gmes_batch::JobSet& js, const std::
     btime_begin(b, tm, e, st);
     launch_main_batch<T>(electric, kBatch3dPP, job_ptr<MainJob<T>>(b, js.main3d_pp[e].off), js.main3d_pp[e].njobs,
                          js.main3d_pp[e].nblocks, st);
+    launch_main_batch<T>(electric, kBatch3dUR, job_ptr<MainJob<T>>(b, js.main3d_ur[e].off), js.main3d_ur[e].njobs,
+                         js.main3d_ur[e].nblocks, st);
     launch_main_batch<T>(electric, js.lds3d[e] ? kBatch3dLds : kBatch3d, job_ptr<MainJob<T>>(b, js.main3d[e].off),
                          js.main3d[e].njobs,
                          js.main3d[e].nblocks, st);
@@ -2254,7 +2300,7 @@ int gmes_batch_step(gmes_batch* b, int64_t nsteps) {
   std::vector<int64_t> sig;
   for (const gmes_batch::JobSet* js : {&b->set, &b->aux_set}) {
     for (int e = 0; e < 2; ++e)
-      for (const auto* L : {&js->main3d[e], &js->main3d_pp[e], &js->main2z[e], &js->mainnd[e], &js->src[e],
+      for (const auto* L : {&js->main3d[e], &js->main3d_pp[e], &js->main3d_ur[e], &js->main2z[e], &js->mainnd[e], &js->src[e],
                             &js->ghost[e]}) {
         sig.push_back((int64_t)L->off);
         sig.push_back(L->njobs);

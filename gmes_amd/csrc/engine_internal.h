@@ -85,6 +85,7 @@ struct gmes_sim {
   uint8_t* ids8[6] = {};     // byte copies for the 3-D main kernels (at most 256 records)
   int32_t prun[3] = {};      // per axis: run (lo | hi << 16) where every CPML profile's c is zero
   bool pp_ok[2] = {false, false};  // E, H: the 3-D FP32 kernels may prefetch the set-0 CPML rows
+  bool ur_ok[2] = {false, false};  // E, H: the same for the set-0 UPML rows
   int64_t prof_off0[6][3] = {};  // pool offsets of profile set 0 (component, axis)
   uint32_t upml_pre = 0;     // components whose UPML state the 3-D kernels prefetch (bit c)
   int32_t ubox[3] = {};      // per axis (lo | hi << 16): a box holding no UPML cell of those

@@ -83,6 +83,14 @@ struct MainArgs {
   // holds no UPML cell of those components
   unsigned upml_mask;
   int ubox[3];
+  // real-FP32 3-D kernels whose three components of the field type all
+  // carry prefetched UPML state: ur = 1 when, per axis, the two components
+  // that read it as a curl axis have identical set-0 rows; urs[a] is then
+  // the pool offset of that row and uro[a] that of the owning component's
+  // row, and the kernels load both with the cell's other operands (lanes
+  // outside ubox)
+  int ur;
+  int urs[3], uro[3];
   // 2-D modes with a collapsed z axis (TMz): 1 when the E kernel writes the
   // z ghost plane (k = 0) of Hx and Hy, which no E update reads, in place
   // of that part of the H ghost copy; hzph: the copy's phase along z for
@@ -225,8 +233,9 @@ constexpr int k2zRows = 1;
 // all fit the LDS staging (RecStage::use_lds for 256-thread blocks)
 // kBatch3dPP: real-FP32 3-D jobs whose records fit the LDS staging and
 // whose CPML profile rows the kernel loads with the other operands
-// (MainArgs::pp)
-constexpr int kBatchGeneric = 0, kBatch3d = 1, kBatch2z = 2, kBatch3dLds = 3, kBatch2zLds = 4, kBatch3dPP = 5;
+// (MainArgs::pp); kBatch3dUR: the same for UPML rows (MainArgs::ur)
+constexpr int kBatchGeneric = 0, kBatch3d = 1, kBatch2z = 2, kBatch3dLds = 3, kBatch2zLds = 4, kBatch3dPP = 5,
+              kBatch3dUR = 6;
 template <class T>
 void launch_main_batch(bool electric, int kind, const MainJob<T>* jobs, int njobs, int64_t nblocks,
                        hipStream_t st);

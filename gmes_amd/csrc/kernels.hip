@@ -190,9 +190,14 @@ __device__ __forceinline__ bool psi_live(const DevRec<R>& rec, const int ix[3]) 
 // the zero-c run of the axis.
 struct PRows {
   float v[3][3];
+  // UPML (MainArgs::ur): per axis, c1..c4 of the row the two components
+  // with that curl axis share, and c5, c6 of the row of the component
+  // that owns the axis
+  float us[3][4];
+  float uo[3][2];
 };
 
-template <int C, class T, bool PRE, bool PP = false>
+template <int C, class T, bool PRE, bool PP = false, bool UR = false>
 __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typename real_of<T>::type>& rec,
                                           T* __restrict__ st, int flat, int i, int j, int k, T f, T D1, T D2,
                                           T pre1, T pre2, const PRows& pr) {
@@ -243,15 +248,23 @@ __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typ
       const T t = vsub(vadd(fma_(-k2, b, vscl(k1, a)), psi1), psi2);
       return E ? fma_(rec.c0, t, f) : fma_(-rec.c0, t, f);
     }
-    // UPML
-    const GMES_GLOBAL float* p1 = (const GMES_GLOBAL float*)m.prof + (set0 ? (int64_t)m.po0[C][Ax<C>::a1] : po[Ax<C>::a1]) + ix[Ax<C>::a1] * 6;
-    const GMES_GLOBAL float* p2 = (const GMES_GLOBAL float*)m.prof + (set0 ? (int64_t)m.po0[C][Ax<C>::a2] : po[Ax<C>::a2]) + ix[Ax<C>::a2] * 6;
-    const GMES_GLOBAL float* p0 = (const GMES_GLOBAL float*)m.prof + (set0 ? (int64_t)m.po0[C][Ax<C>::a0] : po[Ax<C>::a0]) + ix[Ax<C>::a0] * 6;
+    // UPML: c1, c2 along a1, c3, c4 along a2, c5, c6 along the own axis
+    float u1, u2, u3, u4, u5, u6;
+    if (UR && set0) {
+      u1 = pr.us[Ax<C>::a1][0]; u2 = pr.us[Ax<C>::a1][1];
+      u3 = pr.us[Ax<C>::a2][2]; u4 = pr.us[Ax<C>::a2][3];
+      u5 = pr.uo[Ax<C>::a0][0]; u6 = pr.uo[Ax<C>::a0][1];
+    } else {
+      const GMES_GLOBAL float* p1 = (const GMES_GLOBAL float*)m.prof + (set0 ? (int64_t)m.po0[C][Ax<C>::a1] : po[Ax<C>::a1]) + ix[Ax<C>::a1] * 6;
+      const GMES_GLOBAL float* p2 = (const GMES_GLOBAL float*)m.prof + (set0 ? (int64_t)m.po0[C][Ax<C>::a2] : po[Ax<C>::a2]) + ix[Ax<C>::a2] * 6;
+      const GMES_GLOBAL float* p0 = (const GMES_GLOBAL float*)m.prof + (set0 ? (int64_t)m.po0[C][Ax<C>::a0] : po[Ax<C>::a0]) + ix[Ax<C>::a0] * 6;
+      u1 = p1[0]; u2 = p1[1]; u3 = p2[2]; u4 = p2[3]; u5 = p0[4]; u6 = p0[5];
+    }
     const T dstore = (PRE && ((m.upml_mask >> C) & 1u)) ? pre1 : gld(st + flat);
     const T cu = vsub(a, b);
-    const T dn = E ? fma_(p1[1], cu, vscl(p1[0], dstore)) : fma_(-p1[1], cu, vscl(p1[0], dstore));
+    const T dn = E ? fma_(u2, cu, vscl(u1, dstore)) : fma_(-u2, cu, vscl(u1, dstore));
     st_nt(st + flat, dn);
-    return fma_(p2[3] * rec.inv_eps, fma_(-p0[5], dstore, vscl(p0[4], dn)), vscl(p2[2], f));
+    return fma_(u4 * rec.inv_eps, fma_(-u6, dstore, vscl(u5, dn)), vscl(u3, f));
   } else {
     const R d1 = m.d[Ax<C>::a1], d2 = m.d[Ax<C>::a2];
     if (cls == GMES_CLS_DIELECTRIC) {
@@ -293,7 +306,7 @@ __device__ __forceinline__ T update_value(const MainArgs<T>& m, const DevRec<typ
 // Update of one cell of component C: Dummy (no-op), Const (src/pw_const.hh
 // :96,198), Dielectric, CPML, UPML.  ADE classes are left to their kernels.
 // f is the cell's old value, loaded by the caller.
-template <int C, class T, bool PRE = false, bool PP = false>
+template <int C, class T, bool PRE = false, bool PP = false, bool UR = false>
 __device__ __forceinline__ T store_cell(const MainArgs<T>& m, const DevRec<typename real_of<T>::type>& rec,
                                         T* __restrict__ F, T* __restrict__ st, int flat, int i, int j, int k,
                                         T f, T D1, T D2, T pre1 = T{}, T pre2 = T{}, const PRows& pr = PRows{}) {
@@ -304,7 +317,7 @@ __device__ __forceinline__ T store_cell(const MainArgs<T>& m, const DevRec<typen
     st_nt(F + flat, v);
     return v;
   }
-  const T v = update_value<C, T, PRE, PP>(m, rec, st, flat, i, j, k, f, D1, D2, pre1, pre2, pr);
+  const T v = update_value<C, T, PRE, PP, UR>(m, rec, st, flat, i, j, k, f, D1, D2, pre1, pre2, pr);
   st_nt(F + flat, v);
   return v;
 }
@@ -603,7 +616,7 @@ struct Planes {
   T o[3][N];     // old values of the three updated components
   uint16_t r[3][N];
   T s[3][2][N];  // prefetched CPML psi words (real FP32)
-  PRows pr;      // prefetched CPML profile rows (real FP32, one plane, MainArgs::pp)
+  PRows pr;      // prefetched CPML or UPML profile rows (real FP32, one plane, MainArgs::pp / ur)
 };
 
 // The CPML profile rows of coordinates x[0..2] (MainArgs::pp): a lane
@@ -627,7 +640,37 @@ __device__ __forceinline__ void prefetch_rows(const MainArgs<T>& m, const int x[
   }
 }
 
-template <class T, int N, bool UP, bool PP = false>
+// The UPML profile rows of a lane outside m.ubox (MainArgs::ur): per axis
+// a, c1..c4 of the shared row at xs[a] and c5, c6 of the own row at xo[a].
+// Lanes inside the box hold no UPML cell and load nothing.
+template <class T>
+__device__ __forceinline__ void prefetch_urows(const MainArgs<T>& m, const int xs[3], const int xo[3], bool out,
+                                               PRows& pr) {
+  if (!out) return;
+  const GMES_GLOBAL float* prof = (const GMES_GLOBAL float*)m.prof;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const GMES_GLOBAL float* ps = prof + m.urs[a] + xs[a] * 6;
+    const GMES_GLOBAL float* po = prof + m.uro[a] + xo[a] * 6;
+    pr.us[a][0] = ps[0];
+    pr.us[a][1] = ps[1];
+    pr.us[a][2] = ps[2];
+    pr.us[a][3] = ps[3];
+    pr.uo[a][0] = po[4];
+    pr.uo[a][1] = po[5];
+  }
+}
+
+// Whether storage coordinates x lie outside the UPML-free box m.ubox.
+template <class T>
+__device__ __forceinline__ bool outside_ubox(const MainArgs<T>& m, const int x[3]) {
+  bool in = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) in = in && x[a] >= (m.ubox[a] & 0xFFFF) && x[a] < (m.ubox[a] >> 16);
+  return !in;
+}
+
+template <class T, int N, bool UP, bool PP = false, bool UR = false>
 __device__ __forceinline__ void load_e(const MainArgs<T>& m, Planes<T, N>& q, int i0, int j, int k, const int* B, const int* P,
                                        const int* W, const T* __restrict__ hx, const T* __restrict__ hy,
                                        const T* __restrict__ hz, const T* __restrict__ ex, const T* __restrict__ ey,
@@ -665,13 +708,14 @@ __device__ __forceinline__ void load_e(const MainArgs<T>& m, Planes<T, N>& q, in
       prefetch_psi<1, T, UP>(m, fy, ix, q.s[1][0][n], q.s[1][1][n]);
       prefetch_psi<2, T, UP>(m, fz, ix, q.s[2][0][n], q.s[2][1][n]);
       if constexpr (PP) prefetch_rows<T>(m, ix, q.pr);  // E: x at i, y at j, z at k
+      if constexpr (UR) prefetch_urows<T>(m, ix, ix, outside_ubox(m, ix), q.pr);  // E: all at (i, j, k)
     }
   }
 }
 
 // prev: H'z(i0-1,j,k) = Hz[i0, j+1, k] and H'y(i0-1,j,k) = Hy[i0, j, k+1],
 // the i-1 operands of the first plane; advanced to the last plane's.
-template <class T, int N, bool PP = false>
+template <class T, int N, bool PP = false, bool UR = false>
 __device__ __forceinline__ void update_e(const MainArgs<T>& m, const DevRec<typename real_of<T>::type>* recs,
                                          const Planes<T, N>& q, int i0, int j, int k, const int* B, const int* P,
                                          const int* W, T* __restrict__ ex, T* __restrict__ ey, T* __restrict__ ez,
@@ -683,13 +727,13 @@ __device__ __forceinline__ void update_e(const MainArgs<T>& m, const DevRec<type
     const int i = i0 + n;
     const int fx = B[0] + i * P[0] + j * W[0] + k, fy = B[1] + i * P[1] + j * W[1] + k;
     const int fz = B[2] + i * P[2] + j * W[2] + k;
-    const T vx = store_cell<0, T, kPre, PP>(m, recs[q.r[0][n]], ex, sx, fx, i, j, k, q.o[0][n],
+    const T vx = store_cell<0, T, kPre, PP, UR>(m, recs[q.r[0][n]], ex, sx, fx, i, j, k, q.o[0][n],
                                             vsub(q.c[5][n], q.c[6][n]), vsub(q.c[3][n], q.c[4][n]),
                                             q.s[0][0][n], q.s[0][1][n], q.pr);
-    const T vy = store_cell<1, T, kPre, PP>(m, recs[q.r[1][n]], ey, sy, fy, i, j, k, q.o[1][n],
+    const T vy = store_cell<1, T, kPre, PP, UR>(m, recs[q.r[1][n]], ey, sy, fy, i, j, k, q.o[1][n],
                                             vsub(q.c[0][n], q.c[2][n]), vsub(q.c[5][n], hz_prev),
                                             q.s[1][0][n], q.s[1][1][n], q.pr);
-    const T vz = store_cell<2, T, kPre, PP>(m, recs[q.r[2][n]], ez, sz, fz, i, j, k, q.o[2][n],
+    const T vz = store_cell<2, T, kPre, PP, UR>(m, recs[q.r[2][n]], ez, sz, fz, i, j, k, q.o[2][n],
                                             vsub(q.c[3][n], hy_prev), vsub(q.c[0][n], q.c[1][n]),
                                             q.s[2][0][n], q.s[2][1][n], q.pr);
     if (m.fuse && (i == 0 || j == 0 || k == 0)) {
@@ -703,7 +747,7 @@ __device__ __forceinline__ void update_e(const MainArgs<T>& m, const DevRec<type
   }
 }
 
-template <class T, int N, bool UP, bool PP = false>
+template <class T, int N, bool UP, bool PP = false, bool UR = false>
 __device__ __forceinline__ void load_h(const MainArgs<T>& m, Planes<T, N>& q, int i0, int j, int k, const int* B, const int* P,
                                        const int* W, const T* __restrict__ ex, const T* __restrict__ ey,
                                        const T* __restrict__ ez, const T* __restrict__ hx, const T* __restrict__ hy,
@@ -745,12 +789,18 @@ __device__ __forceinline__ void load_h(const MainArgs<T>& m, Planes<T, N>& q, in
         const int x[3] = {i + 1, j + 1, k + 1};  // H: x at i+1, y at j+1, z at k+1
         prefetch_rows<T>(m, x, q.pr);
       }
+      if constexpr (UR) {
+        // shared rows at i+1, j+1, k+1; own rows (Hx along x, Hy along y,
+        // Hz along z) at i, j, k
+        const int xs[3] = {i + 1, j + 1, k + 1}, xo[3] = {i, j, k};
+        prefetch_urows<T>(m, xs, xo, outside_ubox(m, ixx) || outside_ubox(m, ixy) || outside_ubox(m, ixz), q.pr);
+      }
     }
   }
 }
 
 // cur: Ey(i0,j,k), Ez(i0,j,k), advanced to the last plane's i+1 values.
-template <class T, int N, bool PP = false>
+template <class T, int N, bool PP = false, bool UR = false>
 __device__ __forceinline__ void update_h(const MainArgs<T>& m, const DevRec<typename real_of<T>::type>* recs,
                                          const Planes<T, N>& q, int i0, int j, int k, const int* B, const int* P,
                                          const int* W, T* __restrict__ hx, T* __restrict__ hy, T* __restrict__ hz,
@@ -763,13 +813,13 @@ __device__ __forceinline__ void update_h(const MainArgs<T>& m, const DevRec<type
     const int fx = B[3] + i * P[3] + (j + 1) * W[3] + k + 1;
     const int fy = B[4] + (i + 1) * P[4] + j * W[4] + k + 1;
     const int fz = B[5] + (i + 1) * P[5] + (j + 1) * W[5] + k;
-    const T vx = store_cell<3, T, kPre, PP>(m, recs[q.r[0][n]], hx, sx, fx, i, j + 1, k + 1, q.o[0][n],
+    const T vx = store_cell<3, T, kPre, PP, UR>(m, recs[q.r[0][n]], hx, sx, fx, i, j + 1, k + 1, q.o[0][n],
                                             vsub(q.c[5][n], ez_cur), vsub(q.c[3][n], ey_cur), q.s[0][0][n],
                                             q.s[0][1][n], q.pr);
-    const T vy = store_cell<4, T, kPre, PP>(m, recs[q.r[1][n]], hy, sy, fy, i + 1, j, k + 1, q.o[1][n],
+    const T vy = store_cell<4, T, kPre, PP, UR>(m, recs[q.r[1][n]], hy, sy, fy, i + 1, j, k + 1, q.o[1][n],
                                             vsub(q.c[1][n], q.c[0][n]), vsub(q.c[6][n], ez_cur),
                                             q.s[1][0][n], q.s[1][1][n], q.pr);
-    const T vz = store_cell<5, T, kPre, PP>(m, recs[q.r[2][n]], hz, sz, fz, i + 1, j + 1, k, q.o[2][n],
+    const T vz = store_cell<5, T, kPre, PP, UR>(m, recs[q.r[2][n]], hz, sz, fz, i + 1, j + 1, k, q.o[2][n],
                                             vsub(q.c[4][n], ey_cur), vsub(q.c[2][n], q.c[0][n]),
                                             q.s[2][0][n], q.s[2][1][n], q.pr);
     ey_cur = q.c[4][n];
@@ -787,10 +837,12 @@ __device__ __forceinline__ void update_h(const MainArgs<T>& m, const DevRec<type
 // LDSO: the launch guarantees that every block stages its records in LDS
 // (RecStage::use_lds), so one copy reads them from LDS only (batched
 // kernel, 7% faster than the generic pointer).  UP: see prefetch_psi.
-template <bool E, class T, int N, bool SPLIT, bool UP, bool LDSO = false, bool PP = false>
+template <bool E, class T, int N, bool SPLIT, bool UP, bool LDSO = false, bool PP = false, bool UR = false>
 __device__ __forceinline__ void main3d_tile(const MainArgs<T>& m, DevRec<typename real_of<T>::type>* srec, int k,
                                             int j, int i0, int tid, int nthreads) {
   static_assert(!PP || (N == 1 && LDSO && std::is_same<T, float>::value), "row prefetch: real FP32, one plane, LDS records");
+  static_assert(!UR || (N == 1 && LDSO && UP && !PP && std::is_same<T, float>::value),
+                "UPML row prefetch: real FP32, one plane, LDS records, UPML state prefetch");
   using R = typename real_of<T>::type;
   const int nx = m.n[0], ny = m.n[1], nz = m.n[2];
   const bool act = k < nz && j < ny;  // every thread reaches the barrier
@@ -816,11 +868,11 @@ __device__ __forceinline__ void main3d_tile(const MainArgs<T>& m, DevRec<typenam
     T u0 = T{}, u1 = T{};
     if (act) {
       if constexpr (E) {
-        load_e<T, N, UP, PP>(m, q, i0, j, k, B, P, W, f[3], f[4], f[5], f[0], f[1], f[2], ida, idb, idc, i8a, i8b, i8c, m.narrow != 0);
+        load_e<T, N, UP, PP, UR>(m, q, i0, j, k, B, P, W, f[3], f[4], f[5], f[0], f[1], f[2], ida, idb, idc, i8a, i8b, i8c, m.narrow != 0);
         u0 = gld(f[5] + B[5] + i0 * P[5] + (j + 1) * W[5] + k);
         u1 = gld(f[4] + B[4] + i0 * P[4] + j * W[4] + k + 1);
       } else {
-        load_h<T, N, UP, PP>(m, q, i0, j, k, B, P, W, f[0], f[1], f[2], f[3], f[4], f[5], ida, idb, idc, i8a, i8b, i8c, m.narrow != 0);
+        load_h<T, N, UP, PP, UR>(m, q, i0, j, k, B, P, W, f[0], f[1], f[2], f[3], f[4], f[5], ida, idb, idc, i8a, i8b, i8c, m.narrow != 0);
         u0 = gld(f[1] + B[1] + i0 * P[1] + j * W[1] + k);
         u1 = gld(f[2] + B[2] + i0 * P[2] + j * W[2] + k);
       }
@@ -828,8 +880,8 @@ __device__ __forceinline__ void main3d_tile(const MainArgs<T>& m, DevRec<typenam
     stage.commit(srec, m.nrec);
     if (!act) return;
     if constexpr (LDSO) {
-      if constexpr (E) update_e<T, N, PP>(m, srec, q, i0, j, k, B, P, W, f[0], f[1], f[2], sx, sy, sz, u0, u1);
-      else update_h<T, N, PP>(m, srec, q, i0, j, k, B, P, W, f[3], f[4], f[5], sx, sy, sz, u0, u1);
+      if constexpr (E) update_e<T, N, PP, UR>(m, srec, q, i0, j, k, B, P, W, f[0], f[1], f[2], sx, sy, sz, u0, u1);
+      else update_h<T, N, PP, UR>(m, srec, q, i0, j, k, B, P, W, f[3], f[4], f[5], sx, sy, sz, u0, u1);
     } else if constexpr (!SPLIT) {
       const DevRec<R>* recs = stage.use_lds ? srec : grec;
       if constexpr (E) update_e<T, N>(m, recs, q, i0, j, k, B, P, W, f[0], f[1], f[2], sx, sy, sz, u0, u1);
@@ -882,11 +934,11 @@ template <> struct MainWaves<double, 1, true> { static constexpr int v = 7; };
 // LDSO: the launch guarantees the record table fits the LDS staging (see
 // main3d_tile); otherwise the update code is split into an LDS and a
 // global copy.
-template <bool E, class T, int N, bool UP, bool LDSO, bool PP = false>
+template <bool E, class T, int N, bool UP, bool LDSO, bool PP = false, bool UR = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MainWaves<T, N, LDSO>::v)))
 k_main3d(MainArgs<T> m) {
   __shared__ DevRec<typename real_of<T>::type> srec[kRecLds];
-  main3d_tile<E, T, N, !LDSO, UP, LDSO, PP>(m, srec, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
+  main3d_tile<E, T, N, !LDSO, UP, LDSO, PP, UR>(m, srec, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
                        blockIdx.z * N, threadIdx.y * blockDim.x + threadIdx.x, blockDim.x * blockDim.y);
 }
 
@@ -895,10 +947,13 @@ k_main3d(MainArgs<T> m) {
 // a job table in device memory holds each member's arguments and the index
 // of its first block; a block finds its job by binary search on that index
 // (uniform across the block) and runs the same tile body as the single-sim
-// kernel, with the member's own tile shape.
+// kernel (MainJob: which points a block takes).
 
-template <bool E, class T, bool LDSO, bool PP = false>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MainWaves<T, 1>::v)))
+// UR: the UPML rows need more registers than 8 waves per SIMD allow (64
+// VGPRs: 26 to 40 spilled); at 7 waves the kernel fits in 66.
+constexpr int kUrWaves = 7;
+template <bool E, class T, bool LDSO, bool PP = false, bool UR = false>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UR ? kUrWaves : MainWaves<T, 1>::v)))
 k_main3d_batch(const MainJob<T>* jobs, int njobs) {
   __shared__ DevRec<typename real_of<T>::type> srec[kRecLds];
   const MainJob<T>& J = jobs[find_job(jobs, njobs, blockIdx.x)];
@@ -908,7 +963,7 @@ k_main3d_batch(const MainJob<T>* jobs, int njobs) {
   const unsigned j = f / (unsigned)J.m.n[2];
   // PP members have CPML state in every component of the field type, so no
   // UPML state word to prefetch (UP = false, the shorter form)
-  main3d_tile<E, T, 1, false, !PP, LDSO, PP>(J.m, srec, (int)(f - j * (unsigned)J.m.n[2]), (int)j, (int)i,
+  main3d_tile<E, T, 1, false, !PP, LDSO, PP, UR>(J.m, srec, (int)(f - j * (unsigned)J.m.n[2]), (int)j, (int)i,
                                               threadIdx.x, 256);
 }
 
@@ -1428,6 +1483,12 @@ void launch_main3d(bool electric, const MainArgs<T>& m, int bx, int by, int ichu
   }
   if constexpr (std::is_same<T, float>::value) {
     if (m.upml_mask) {
+      // UPML profile rows loaded with the other operands (MainArgs::ur)
+      if (m.ur && lds && n == 1) {
+        if (electric) hipLaunchKernelGGL((k_main3d<true, T, 1, true, true, false, true>), grid, block, 0, st, m);
+        else hipLaunchKernelGGL((k_main3d<false, T, 1, true, true, false, true>), grid, block, 0, st, m);
+        return;
+      }
       if (lds) GMES_L3(1, true, true)
       GMES_L3(1, true, false) GMES_L3(2, true, false) GMES_L3(4, true, false) GMES_L3(8, true, false)
     }
@@ -1468,6 +1529,11 @@ void launch_main_batch(bool electric, int kind, const MainJob<T>* jobs, int njob
     if constexpr (std::is_same<T, float>::value) {
       if (electric) hipLaunchKernelGGL((k_main3d_batch<true, T, true, true>), grid, block, 0, st, jobs, njobs);
       else hipLaunchKernelGGL((k_main3d_batch<false, T, true, true>), grid, block, 0, st, jobs, njobs);
+    }
+  } else if (kind == kBatch3dUR) {
+    if constexpr (std::is_same<T, float>::value) {
+      if (electric) hipLaunchKernelGGL((k_main3d_batch<true, T, true, false, true>), grid, block, 0, st, jobs, njobs);
+      else hipLaunchKernelGGL((k_main3d_batch<false, T, true, false, true>), grid, block, 0, st, jobs, njobs);
     }
   } else if (kind == kBatch3dLds) {
     if (electric) hipLaunchKernelGGL((k_main3d_batch<true, T, true>), grid, block, 0, st, jobs, njobs);
