@@ -285,7 +285,7 @@ def batched_leg(steps, warm=5, alone_steps=50, prof_steps=50):
     roof = dict(bound="hbm", achieved=fam[dom]["gbps"], peak=HBM_PEAK_GBPS, unit="GB/s",
                 frac=round(fam[dom]["gbps"] / HBM_PEAK_GBPS, 4),
                 traffic=measured_traffic_batch(dom),
-                kernel="k_main3d_batch<%s, float> (%d 3-D members)" % (
+                kernel="k_main3d_batch<%s, float> (%d 3-D members, one launch per profile-row kind)" % (
                     "true" if dom == "e_main3d" else "false", len(three)),
                 bytes_per_launch=fam[dom]["bytes_per_launch"], avg_launch_ms=fam[dom]["avg_launch_ms"],
                 families=fam)
@@ -460,23 +460,40 @@ def served_leg(nreq, budget_gb=64.0, max_active=16, chunk=250, policy="shortest"
 
 
 def copy_gbps(reps=20):
-    """Read-plus-write rate of torch's device copy of 256 MiB into 256 MiB
-    (HIP events): the rate a plain stream reaches on this part, beside the
-    8 TB/s nominal peak."""
-    import torch
-    src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
-    for _ in range(3):
-        dst.copy_(src)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        dst.copy_(src)
-    e1.record()
-    e1.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    del src, dst
-    return 2 * (64 << 20) * 4 / (ms * 1e-3) / 1e9
+    """Read-plus-write rate of a device-to-device hipMemcpyAsync of 256 MiB
+    into 256 MiB (HIP events on the null stream; the HIP runtime the engine
+    links): the rate a plain stream reaches on this part, beside the 8 TB/s
+    nominal peak.  Plain HIP calls keep it usable under a kernel tracer."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    n = ctypes.c_size_t(256 << 20)
+    a, b, e0, e1 = (ctypes.c_void_p() for _ in range(4))
+
+    def ok(rc):
+        if rc != 0:
+            raise RuntimeError("HIP error %d in the copy probe" % rc)
+
+    ok(hip.hipMalloc(ctypes.byref(a), n))
+    ok(hip.hipMalloc(ctypes.byref(b), n))
+    try:
+        ok(hip.hipMemset(a, 1, n))
+        ok(hip.hipEventCreate(ctypes.byref(e0)))
+        ok(hip.hipEventCreate(ctypes.byref(e1)))
+        for _ in range(3):
+            ok(hip.hipMemcpyAsync(b, a, n, 3, None))  # 3: hipMemcpyDeviceToDevice
+        ok(hip.hipEventRecord(e0, None))
+        for _ in range(reps):
+            ok(hip.hipMemcpyAsync(b, a, n, 3, None))
+        ok(hip.hipEventRecord(e1, None))
+        ok(hip.hipEventSynchronize(e1))
+        ms = ctypes.c_float()
+        ok(hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1))
+        ok(hip.hipEventDestroy(e0))
+        ok(hip.hipEventDestroy(e1))
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    return 2 * n.value / (ms.value / reps * 1e-3) / 1e9
 
 
 def copy_probe():
@@ -559,7 +576,7 @@ def main():
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBPS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBPS, 4),
                     traffic=measured_traffic(n, "E" if fam == 0 else "H"),
-                    kernel="k_main3d<%s, float, 1, false, true>" % ("true" if fam == 0 else "false"),
+                    kernel="k_main3d<%s, float, 1, false, true, true, false>" % ("true" if fam == 0 else "false"),
                     bytes_per_launch=int(nbytes), avg_launch_ms=round(avg_ms, 5),
                     other_kernel_gbps=round(other_gbps, 1))
     cg = copy_gbps()
