@@ -345,10 +345,11 @@ def test_fp32_3d_with_more_than_256_records(engine_lib, oracle_lib):
 @pytest.mark.parametrize("bound", [g.Cpml, g.Upml], ids=["cpml", "upml"])
 def test_fp32_tile_shapes_give_the_same_bits(engine_lib, bound):
     """Real FP32, 3-D: the default 128 x 2 tile stages the records in LDS
-    and, with CPML on every face, loads the profile rows with the other
-    operands; a 16 x 1 tile is too small to stage the 9 records (27
-    16-byte chunks) and reads them, and the rows, after each cell's
-    record.  Same arithmetic per cell: the fields must agree bit for bit."""
+    and, with CPML (or UPML) on every face, loads the CPML (or UPML)
+    profile rows with the other operands; a 16 x 1 tile is too small to
+    stage the 9 records (27 16-byte chunks) and reads them, and the rows,
+    after each cell's record.  Same arithmetic per cell: the fields must
+    agree bit for bit."""
     out = []
     for tile in (None, (16, 1)):
         sim = spheres3d(engine_lib, precision="fp32", n=40, bound=bound)
