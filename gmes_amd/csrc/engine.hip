@@ -575,7 +575,10 @@ void tile_of(const gmes_sim* s, const int ext[3], int& bx, int& by) {
   by = s->tile_by;
   if (bx > 0 && by > 0 && (is3d(s) || (bx <= next_pow2(ext[2]) && ext[2] > 2))) return;
   if (is3d(s)) {
-    bx = std::min(128, next_pow2(s->g.nz));
+    // real FP32: whole 256-cell rows per block when they divide the row
+    // (256^3: E launch 0.1708 to 0.1675 ms, H unchanged; FP64 0.5% slower
+    // that way), else 128 x 2
+    bx = s->esize == 4 && s->g.nz % 256 == 0 ? 256 : std::min(128, next_pow2(s->g.nz));
     by = std::max(1, 256 / bx);
   } else if (ext[2] <= 2) {
     bx = 1;  // short rows: a thread per (i, j) walks k (MainArgs::kloop)
