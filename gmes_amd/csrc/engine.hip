@@ -568,13 +568,17 @@ bool main_args(const gmes_sim* s, bool electric, MainArgs<T>& m, int ext[3]) {
 
 // Tile (bx along k, by along j) of the main kernels: the sim's own setting
 // (gmes_sim_set_tile, gmes_sim_autotune) or the default.  3-D: (j,k) tiles
-// of one i plane, 128 x 2 for long rows.  Other modes: a k x j tile of one
-// i plane.
+// of one i plane, 128 x 2 for long rows; on packed rows (gmes_sim::packed)
+// 256 x 1 means flat blocks of 256 consecutive (j, k) points of a plane,
+// the default there.  Other modes: a k x j tile of one i plane.
 void tile_of(const gmes_sim* s, const int ext[3], int& bx, int& by) {
   bx = s->tile_bx;
   by = s->tile_by;
   if (bx > 0 && by > 0 && (is3d(s) || (bx <= next_pow2(ext[2]) && ext[2] > 2))) return;
-  if (is3d(s)) {
+  if (is3d(s) && s->packed) {
+    bx = 256;
+    by = 1;
+  } else if (is3d(s)) {
     // real FP32: whole 256-cell rows per block when they divide the row
     // (256^3: E launch 0.1708 to 0.1675 ms, H unchanged; FP64 0.5% slower
     // that way), else 128 x 2
@@ -592,7 +596,9 @@ void tile_of(const gmes_sim* s, const int ext[3], int& bx, int& by) {
 template <class T>
 void launch_main_tile(const gmes_sim* s, bool electric, const MainArgs<T>& m, int ext[3], int bx, int by,
                       hipStream_t st) {
-  if (is3d(s)) {
+  if (is3d(s) && s->packed && bx == 256 && by == 1) {
+    launch_main3d_flat<T>(electric, m, st);
+  } else if (is3d(s)) {
     launch_main3d<T>(electric, m, bx, by, s->tile_ic > 0 ? s->tile_ic : 1, st);
   } else {
     launch_main<T>(electric, m, ext, bx, by, st);
@@ -951,6 +957,7 @@ int autotune_t(gmes_sim* s, int& best_bx, int& best_by) {
   std::vector<int> cand;
   const int row = next_pow2(ext[2]);
   for (int bx = 1; bx <= 256; bx *= 2) {
+    if (bx == 256 && s->packed) { cand.push_back(bx); continue; }  // flat blocks
     if (bx > std::max(row, 1) && bx > 1) continue;          // wider than a row: idle lanes
     if (row >= 32 && bx < 32) continue;                      // narrower than a 128-byte line
     cand.push_back(bx);
@@ -1162,6 +1169,12 @@ int gmes_sim_create(const gmes_grid_desc* desc, gmes_sim** out) {
   s->fp64 = desc->dtype == GMES_F64 || desc->dtype == GMES_C128;
   s->esize = (s->fp64 ? 8 : 4) * (s->complex_ ? 2 : 1);
   const int64_t n[3] = {desc->nx, desc->ny, desc->nz};
+  // Real-FP32 3-D sims with rows under 256 cells store their rows back to
+  // back: with 128-byte lines, padding short rows to 32 words left a part-
+  // used line at every row end (the batched 64-instance workload's 3-D
+  // main launches: 1.112 to 1.049 ms (E), 1.058 to 1.002 ms (H) unpadded);
+  // their single-sim launches then take flat blocks (tile_of).
+  s->packed = desc->active == 0x3Fu && desc->dtype == GMES_F32 && desc->nz < 256;
   for (int c = 0; c < 6; ++c) {
     if (!active(s, c)) {
       s->shape[c][0] = s->shape[c][1] = s->shape[c][2] = 1;
@@ -1171,7 +1184,7 @@ int gmes_sim_create(const gmes_grid_desc* desc, gmes_sim** out) {
     }
     s->ncell[c] = s->shape[c][0] * s->shape[c][1] * s->shape[c][2];
     const int64_t s2 = s->shape[c][2];
-    const bool padded = s2 >= 32;
+    const bool padded = s2 >= 32 && !s->packed;
     s->pitch[c] = (int32_t)(padded ? (s2 + 31) / 32 * 32 : s2);
     s->koff[c] = padded && (c == GMES_HX || c == GMES_HY) ? 1 : 0;
     s->dsize[c] = 2 * kPad + s->shape[c][0] * s->shape[c][1] * s->pitch[c];

@@ -942,6 +942,26 @@ k_main3d(MainArgs<T> m) {
                        blockIdx.z * N, threadIdx.y * blockDim.x + threadIdx.x, blockDim.x * blockDim.y);
 }
 
+// UR in the flat and batched kernels: the UPML rows need more registers
+// than 8 waves per SIMD allow (64 VGPRs: 26 to 40 spilled); at 7 waves the
+// kernels fit in 66.
+constexpr int kUrWaves = 7;
+
+// Flat single-sim launch (packed rows, gmes_sim::packed): block b takes the
+// 256 consecutive row-major (j, k) points b % gx of plane b / gx, with
+// gx = ceil(ny * nz / 256), so short rows leave no lane idle.
+template <bool E, class T, bool UP, bool LDSO, bool PP = false, bool UR = false>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UR ? kUrWaves : MainWaves<T, 1, LDSO>::v)))
+k_main3d_flat(MainArgs<T> m) {
+  __shared__ DevRec<typename real_of<T>::type> srec[kRecLds];
+  const unsigned nz = (unsigned)m.n[2];
+  const unsigned gx = ((unsigned)m.n[1] * nz + 255u) / 256u;
+  const unsigned i = blockIdx.x / gx;
+  const unsigned f = (blockIdx.x - i * gx) * 256u + threadIdx.x;
+  const unsigned j = f / nz;
+  main3d_tile<E, T, 1, !LDSO, UP, LDSO, PP, UR>(m, srec, (int)(f - j * nz), (int)j, (int)i, threadIdx.x, 256);
+}
+
 // ------------------------------------------------------------ batched launches
 // Many sims of one storage type advance with one launch per kernel kind:
 // a job table in device memory holds each member's arguments and the index
@@ -949,9 +969,6 @@ k_main3d(MainArgs<T> m) {
 // (uniform across the block) and runs the same tile body as the single-sim
 // kernel (MainJob: which points a block takes).
 
-// UR: the UPML rows need more registers than 8 waves per SIMD allow (64
-// VGPRs: 26 to 40 spilled); at 7 waves the kernel fits in 66.
-constexpr int kUrWaves = 7;
 template <bool E, class T, bool LDSO, bool PP = false, bool UR = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UR ? kUrWaves : MainWaves<T, 1>::v)))
 k_main3d_batch(const MainJob<T>* jobs, int njobs) {
@@ -1505,6 +1522,35 @@ void launch_main3d(bool electric, const MainArgs<T>& m, int bx, int by, int ichu
 }
 
 template <class T>
+void launch_main3d_flat(bool electric, const MainArgs<T>& m, hipStream_t st) {
+  if constexpr (std::is_same<T, float>::value) {
+    const unsigned gx = ((unsigned)m.n[1] * (unsigned)m.n[2] + 255u) / 256u;
+    const dim3 grid(gx * (unsigned)m.n[0]), block(256);
+    constexpr int chunks = (int)(sizeof(DevRec<float>) / 16);
+    const bool lds = m.nrec <= kRecLds && m.nrec * chunks <= 256;
+#define GMES_LF(UP, LDSO, PP, UR)                                                                 \
+  {                                                                                               \
+    if (electric) hipLaunchKernelGGL((k_main3d_flat<true, T, UP, LDSO, PP, UR>), grid, block, 0, st, m); \
+    else hipLaunchKernelGGL((k_main3d_flat<false, T, UP, LDSO, PP, UR>), grid, block, 0, st, m);        \
+    return;                                                                                       \
+  }
+    if (m.upml_mask) {
+      if (lds && m.ur) GMES_LF(true, true, false, true)
+      if (lds) GMES_LF(true, true, false, false)
+      GMES_LF(true, false, false, false)
+    }
+    if (m.pp && lds) GMES_LF(false, true, true, false)
+    if (lds) GMES_LF(false, true, false, false)
+    GMES_LF(false, false, false, false)
+#undef GMES_LF
+  } else {
+    (void)electric;  // packed rows are real FP32 only
+    (void)m;
+    (void)st;
+  }
+}
+
+template <class T>
 void launch_ade(int cls, int c, const AdeArgs<T>& m, hipStream_t st) {
   dim3 block(256), grid((unsigned)((m.n + 255) / 256));
 #define GMES_ADE_SWITCH(KERNEL)                                               \
@@ -1648,6 +1694,7 @@ void launch_sum_sq(const SqBox<T>& b, double* partial, hipStream_t st) {
   template void launch_sum_sq<T>(const SqBox<T>&, double*, hipStream_t);                        \
   template void launch_main<T>(bool, const MainArgs<T>&, int[3], int, int, hipStream_t);       \
   template void launch_main3d<T>(bool, const MainArgs<T>&, int, int, int, hipStream_t);        \
+  template void launch_main3d_flat<T>(bool, const MainArgs<T>&, hipStream_t);                 \
   template void launch_main2z<T>(bool, const MainArgs<T>&, int[3], hipStream_t);               \
   template void launch_ade<T>(int, int, const AdeArgs<T>&, hipStream_t);                       \
   template void launch_src<T>(const SrcArgs<T>&, hipStream_t);                                  \
