@@ -1169,12 +1169,13 @@ int gmes_sim_create(const gmes_grid_desc* desc, gmes_sim** out) {
   s->fp64 = desc->dtype == GMES_F64 || desc->dtype == GMES_C128;
   s->esize = (s->fp64 ? 8 : 4) * (s->complex_ ? 2 : 1);
   const int64_t n[3] = {desc->nx, desc->ny, desc->nz};
-  // Real-FP32 3-D sims with rows under 256 cells store their rows back to
-  // back: with 128-byte lines, padding short rows to 32 words left a part-
-  // used line at every row end (the batched 64-instance workload's 3-D
-  // main launches: 1.112 to 1.049 ms (E), 1.058 to 1.002 ms (H) unpadded);
-  // their single-sim launches then take flat blocks (tile_of).
-  s->packed = desc->active == 0x3Fu && desc->dtype == GMES_F32 && desc->nz < 256;
+  // 3-D sims with rows under 256 cells store their rows back to back:
+  // with 128-byte lines, padding short rows to 32 words left a part-used
+  // line at every row end (the batched 64-instance workload's 3-D main
+  // launches: 1.112 to 1.049 ms (E), 1.058 to 1.002 ms (H) unpadded; the
+  // c64 dispersive cell's E launch 0.558 to 0.491 ms); their single-sim
+  // launches then take flat blocks (tile_of).
+  s->packed = desc->active == 0x3Fu && desc->nz < 256;
   for (int c = 0; c < 6; ++c) {
     if (!active(s, c)) {
       s->shape[c][0] = s->shape[c][1] = s->shape[c][2] = 1;

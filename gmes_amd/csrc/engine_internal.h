@@ -78,7 +78,7 @@ struct gmes_sim {
   int64_t shape[6][3];
   int64_t ncell[6];
   int32_t pitch[6];          // device row pitch (elements)
-  bool packed = false;       // rows unpadded: real-FP32 3-D sims with rows under 256 cells
+  bool packed = false;       // rows unpadded: 3-D sims with rows under 256 cells
   int32_t koff[6];           // device k offset (see FieldArgs)
   int64_t dsize[6];          // device elements per component
   void* f[6] = {};           // device fields

@@ -225,7 +225,7 @@ void launch_sum_sq(const SqBox<T>& b, double* partial, hipStream_t st);
 
 template <class T>
 void launch_main(bool electric, const MainArgs<T>& m, int ext[3], int bx, int by, hipStream_t st);
-// flat blocks over packed rows (real FP32 3-D; see k_main3d_flat)
+// flat blocks over packed rows (3-D; see k_main3d_flat)
 template <class T>
 void launch_main3d_flat(bool electric, const MainArgs<T>& m, hipStream_t st);
 // collapsed-z 2-D kernels: i rows per thread (1 measured fastest of 1, 2

@@ -1544,9 +1544,17 @@ void launch_main3d_flat(bool electric, const MainArgs<T>& m, hipStream_t st) {
     GMES_LF(false, false, false, false)
 #undef GMES_LF
   } else {
-    (void)electric;  // packed rows are real FP32 only
-    (void)m;
-    (void)st;
+    const unsigned gx = ((unsigned)m.n[1] * (unsigned)m.n[2] + 255u) / 256u;
+    const dim3 grid(gx * (unsigned)m.n[0]), block(256);
+    constexpr int chunks = (int)(sizeof(DevRec<typename real_of<T>::type>) / 16);
+    const bool lds = m.nrec <= kRecLds && m.nrec * chunks <= 256;
+    if (lds) {
+      if (electric) hipLaunchKernelGGL((k_main3d_flat<true, T, false, true>), grid, block, 0, st, m);
+      else hipLaunchKernelGGL((k_main3d_flat<false, T, false, true>), grid, block, 0, st, m);
+    } else {
+      if (electric) hipLaunchKernelGGL((k_main3d_flat<true, T, false, false>), grid, block, 0, st, m);
+      else hipLaunchKernelGGL((k_main3d_flat<false, T, false, false>), grid, block, 0, st, m);
+    }
   }
 }
 
