@@ -255,7 +255,10 @@ int gmes_sim_sync(gmes_sim* sim);
 int gmes_sim_get_step(const gmes_sim* sim, int64_t* step);
 /* 1: capture one step as a HIP graph and replay it; 0: launch directly. */
 int gmes_sim_set_graph(gmes_sim* sim, int32_t enable);
-/* Launch-shape override (0 = autotuned default) for the main E/H kernels. */
+/* Launch-shape override (0 = autotuned default) for the main E/H kernels:
+ * a bx x by tile of one plane (bx along k, a power of two, bx * by <= 256).
+ * On a 3-D sim whose rows are under 256 cells (stored unpadded) 256 x 1,
+ * the default there, means flat blocks of 256 consecutive (j, k) points. */
 int gmes_sim_set_tile(gmes_sim* sim, int32_t bx, int32_t by);
 /* Tile autotuner: times the E main kernel under each candidate tile
  * (fields and PML state are restored afterwards), keeps the fastest per
