@@ -91,6 +91,8 @@ const int kGhostOrder[6][2] = {{1, 2}, {2, 0}, {0, 1}, {1, 2}, {2, 0}, {0, 1}};
 
 bool active(const gmes_sim* s, int c) { return (s->g.active >> c) & 1u; }
 bool is3d(const gmes_sim* s) { return s->g.active == 0x3Fu; }
+// 3-D sims with rows under 256 cells store their rows unpadded (gmes_sim::packed)
+bool packed_rows(const gmes_grid_desc* d) { return d->active == 0x3Fu && d->nz < 256; }
 // A 2-D mode with a collapsed z axis (TMz, TEz): every active component
 // has rows of at most 2 elements and exactly one updated z plane (its own
 // axis z has extent 1; a z extent of 2 holds one ghost plane).  Single-sim
@@ -1153,7 +1155,7 @@ int gmes_sim_create(const gmes_grid_desc* desc, gmes_sim** out) {
     if (!((desc->active >> c) & 1u)) continue;
     const int64_t ext[3] = {(int64_t)desc->nx + (c % 3 != 0), (int64_t)desc->ny + (c % 3 != 1),
                             (int64_t)desc->nz + (c % 3 != 2)};
-    const int64_t row = ext[2] >= 32 ? (ext[2] + 31) / 32 * 32 : ext[2];
+    const int64_t row = ext[2] >= 32 && !packed_rows(desc) ? (ext[2] + 31) / 32 * 32 : ext[2];
     const double elems = 2.0 * kPad + (double)ext[0] * (double)ext[1] * (double)row;
     if (elems >= 2147483648.0)
       return set_error(GMES_EINVAL, "component %d would hold %.0f device elements; the limit is 2^31 - 1", c, elems);
@@ -1175,7 +1177,7 @@ int gmes_sim_create(const gmes_grid_desc* desc, gmes_sim** out) {
   // launches: 1.112 to 1.049 ms (E), 1.058 to 1.002 ms (H) unpadded; the
   // c64 dispersive cell's E launch 0.558 to 0.491 ms); their single-sim
   // launches then take flat blocks (tile_of).
-  s->packed = desc->active == 0x3Fu && desc->nz < 256;
+  s->packed = packed_rows(desc);
   for (int c = 0; c < 6; ++c) {
     if (!active(s, c)) {
       s->shape[c][0] = s->shape[c][1] = s->shape[c][2] = 1;

@@ -163,7 +163,7 @@ def test_engine_rejects_grids_beyond_32_bit_offsets():
         pytest.skip("engine not built (run __graft_entry__.build())")
     lib = _abi.Library(_abi.ENGINE_PATH)
     for n, active in (((1290, 1290, 1290), 0x3F), ((2, 2, 1 << 29), 0x3F),
-                      ((46341, 46341, 1), 0x1C)):
+                      ((46341, 46341, 1), 0x1C), ((2901, 2901, 255), 0x3F)):  # the last: unpadded rows
         desc = _abi.GridDesc()
         desc.nx, desc.ny, desc.nz = n
         desc.dtype = _abi.F32
