@@ -28,7 +28,9 @@ constexpr int kPad = 32;  // elements before row 0 of every device array
 // pitch p is s2 rounded up to 32 elements when s2 >= 32 (else s2), and
 // koff = 1 for Hx and Hy on a padded layout, so that the first updated
 // cell of every row (k = 0 for E and Hz, k = 1 for Hx, Hy) starts a
-// 128-byte line.  Material ids and CPML/UPML state use the same layout.
+// 128-byte line.  3-D sims with rows under 256 cells keep p = s2 and
+// koff = 0 (gmes_sim::packed).  Material ids and CPML/UPML state use the
+// same layout.
 template <class T>
 struct FieldArgs {
   T* f[6];
