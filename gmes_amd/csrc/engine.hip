@@ -1248,7 +1248,7 @@ int gmes_sim_destroy(gmes_sim* s) {
     if (d.acc) (void)dev_release(d.acc);
   void* bufs[] = {s->d_recs, s->d_coef, s->d_prof, s->d_prof_off, s->d_ops, s->d_groups, s->d_waves,
                   s->d_aux_ptrs, s->d_probe_cells, s->d_probe_comps, s->d_probe_buf, s->d_clock,
-                  s->d_dft_w, s->d_dft_jobs, s->d_sq};
+                  s->d_dft_w, s->d_dft_jobs, s->d_sq, s->d_stage};
   for (void* b : bufs)
     if (b) (void)dev_release(b);
   if (s->stream) (void)stream_release(s->stream);
@@ -1660,22 +1660,41 @@ int gmes_sim_read_dft(gmes_sim* s, int32_t id, double* out, int64_t count) {
   return GMES_OK;
 }
 
+// The staging buffer of field reads and writes of padded rows, at least
+// `bytes` long (grown on demand; the sim's stream is idle when it grows).
+static int stage_reserve(gmes_sim* s, int64_t bytes) {
+  if (bytes <= s->stage_cap) return GMES_OK;
+  HIP_TRY(sync_exec(s));
+  if (s->d_stage) dev_free(s, s->d_stage, (size_t)s->stage_cap);
+  s->d_stage = nullptr;
+  s->stage_cap = 0;
+  int rc = dev_alloc(s, &s->d_stage, (size_t)bytes);
+  if (rc) return rc;
+  s->stage_cap = bytes;
+  return GMES_OK;
+}
+
 int gmes_sim_write_field(gmes_sim* s, int32_t c, const void* host, int64_t bytes) {
   ApiGuard api;
   if (!s) return set_error(GMES_EINVAL, "null sim handle");
   if (c < 0 || c > 5) return set_error(GMES_EINVAL, "bad component");
   if (bytes != s->ncell[c] * s->esize) return set_error(GMES_EINVAL, "size mismatch");
   HIP_TRY(hipSetDevice(s->device));
-  const size_t row = (size_t)s->shape[c][2] * s->esize;
+  HIP_TRY(sync_exec(s));  // steps queued on the sim's (or its batch's) stream come first
   if (s->pitch[c] == s->shape[c][2] && s->koff[c] == 0) {
     HIP_TRY(hipMemcpyAsync(static_cast<char*>(s->f[c]) + (size_t)kPad * s->esize, host, (size_t)bytes,
                            hipMemcpyHostToDevice, s->stream));
     HIP_TRY(sync_exec(s));
     return GMES_OK;
   }
-  HIP_TRY(hipMemcpy2DAsync(static_cast<char*>(s->f[c]) + (size_t)(kPad - s->koff[c]) * s->esize,
-                           (size_t)s->pitch[c] * s->esize, host, row, row,
-                           (size_t)(s->shape[c][0] * s->shape[c][1]), hipMemcpyHostToDevice, s->stream));
+  // one contiguous upload, then rows moved to their pitched places
+  int rc = stage_reserve(s, bytes);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(s->d_stage, host, (size_t)bytes, hipMemcpyHostToDevice, s->stream));
+  launch_pitch_copy(s->d_stage, static_cast<char*>(s->f[c]) + (size_t)(kPad - s->koff[c]) * s->esize,
+                    s->shape[c][0] * s->shape[c][1], (int)s->shape[c][2], (int)s->shape[c][2], s->pitch[c], s->esize,
+                    s->stream);
+  HIP_TRY(hipGetLastError());
   HIP_TRY(sync_exec(s));
   return GMES_OK;
 }
@@ -1686,16 +1705,21 @@ int gmes_sim_read_field(gmes_sim* s, int32_t c, void* host, int64_t bytes) {
   if (c < 0 || c > 5) return set_error(GMES_EINVAL, "bad component");
   if (bytes != s->ncell[c] * s->esize) return set_error(GMES_EINVAL, "size mismatch");
   HIP_TRY(hipSetDevice(s->device));
-  const size_t row = (size_t)s->shape[c][2] * s->esize;
+  HIP_TRY(sync_exec(s));  // steps queued on the sim's (or its batch's) stream come first
   if (s->pitch[c] == s->shape[c][2] && s->koff[c] == 0) {
     HIP_TRY(hipMemcpyAsync(host, static_cast<const char*>(s->f[c]) + (size_t)kPad * s->esize, (size_t)bytes,
                            hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(sync_exec(s));
     return GMES_OK;
   }
-  HIP_TRY(hipMemcpy2DAsync(host, row, static_cast<const char*>(s->f[c]) + (size_t)(kPad - s->koff[c]) * s->esize,
-                           (size_t)s->pitch[c] * s->esize, row, (size_t)(s->shape[c][0] * s->shape[c][1]),
-                           hipMemcpyDeviceToHost, s->stream));
+  // rows packed on the device, then one contiguous download
+  int rc = stage_reserve(s, bytes);
+  if (rc) return rc;
+  launch_pitch_copy(static_cast<const char*>(s->f[c]) + (size_t)(kPad - s->koff[c]) * s->esize, s->d_stage,
+                    s->shape[c][0] * s->shape[c][1], (int)s->shape[c][2], s->pitch[c], (int)s->shape[c][2], s->esize,
+                    s->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(host, s->d_stage, (size_t)bytes, hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(sync_exec(s));
   return GMES_OK;
 }

@@ -168,4 +168,7 @@ struct gmes_sim {
   // partial sums of gmes_sim_field_sum_sq (grown on demand)
   double* d_sq = nullptr;
   int64_t sq_cap = 0;
+  // contiguous staging of one component for reads and writes of padded rows
+  void* d_stage = nullptr;
+  int64_t stage_cap = 0;  // bytes
 };

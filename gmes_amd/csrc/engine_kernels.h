@@ -225,6 +225,10 @@ inline int64_t sq_blocks(int64_t ncells) { return (ncells + kSqBlockCells - 1) /
 template <class T>
 void launch_sum_sq(const SqBox<T>& b, double* partial, hipStream_t st);
 
+// rows x n2 elements of esize bytes from rows spaced spitch elements apart
+// to rows spaced dpitch apart (field reads and writes of padded layouts)
+void launch_pitch_copy(const void* src, void* dst, int64_t rows, int n2, int spitch, int dpitch, int esize,
+                       hipStream_t st);
 template <class T>
 void launch_main(bool electric, const MainArgs<T>& m, int ext[3], int bx, int by, hipStream_t st);
 // flat blocks over packed rows (3-D; see k_main3d_flat)

@@ -1558,6 +1558,39 @@ void launch_main3d_flat(bool electric, const MainArgs<T>& m, hipStream_t st) {
   }
 }
 
+// Row repacking for field reads and writes of padded layouts: element k of
+// row r moves from src[r * sp + k] to dst[r * dp + k].  A 2-D copy from
+// the pitched device rows straight to pageable host memory ran at 0.2 GB/s
+// (256^3 FP32: 0.3 s per component); packing on the device and copying
+// one contiguous block is limited by the host link instead.
+template <class W>
+__global__ void __launch_bounds__(256) k_pitch_copy(const W* __restrict__ src, W* __restrict__ dst, int64_t total,
+                                                    int n2, int sp, int dp) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= total) return;
+  const int64_t r = t / n2;
+  const int64_t k = t - r * n2;
+  dst[r * dp + k] = src[r * sp + k];
+}
+
+void launch_pitch_copy(const void* src, void* dst, int64_t rows, int n2, int spitch, int dpitch, int esize,
+                       hipStream_t st) {
+  const int64_t total = rows * n2;
+  if (total <= 0) return;
+  const dim3 grid((unsigned)((total + 255) / 256)), block(256);
+  typedef unsigned long long u64_t;
+  typedef int w16_t __attribute__((ext_vector_type(4)));
+  if (esize == 4)
+    hipLaunchKernelGGL((k_pitch_copy<uint32_t>), grid, block, 0, st, (const uint32_t*)src, (uint32_t*)dst, total, n2,
+                       spitch, dpitch);
+  else if (esize == 8)
+    hipLaunchKernelGGL((k_pitch_copy<u64_t>), grid, block, 0, st, (const u64_t*)src, (u64_t*)dst, total, n2, spitch,
+                       dpitch);
+  else
+    hipLaunchKernelGGL((k_pitch_copy<w16_t>), grid, block, 0, st, (const w16_t*)src, (w16_t*)dst, total, n2, spitch,
+                       dpitch);
+}
+
 template <class T>
 void launch_ade(int cls, int c, const AdeArgs<T>& m, hipStream_t st) {
   dim3 block(256), grid((unsigned)((m.n + 255) / 256));
