@@ -258,7 +258,10 @@ int gmes_sim_set_graph(gmes_sim* sim, int32_t enable);
 /* Launch-shape override (0 = autotuned default) for the main E/H kernels:
  * a bx x by tile of one plane (bx along k, a power of two, bx * by <= 256).
  * On a 3-D sim whose rows are under 256 cells (stored unpadded) 256 x 1,
- * the default there, means flat blocks of 256 consecutive (j, k) points. */
+ * the default there, means flat blocks of 256 consecutive (j, k) points.
+ * On a padded 3-D sim the environment variable GMES_ICHUNK, read by this
+ * call, sets the i planes per block (1, 2, 4 or 8; default 1), a tuning
+ * knob the measurements keep at 1. */
 int gmes_sim_set_tile(gmes_sim* sim, int32_t bx, int32_t by);
 /* Tile autotuner: times the E main kernel under each candidate tile
  * (fields and PML state are restored afterwards), keeps the fastest per
