@@ -15,11 +15,7 @@ from .fdtd import (FDTD, TEMxFDTD, TEMyFDTD, TEMzFDTD, TExFDTD, TEyFDTD, TEzFDTD
 from .geometry import (Block, Cartesian, Cone, Cylinder, DefaultMedium, Ellipsoid, Shell, Sphere)
 from .material import (Const, Cpml, CriticalPoint, DcpAde, DcpPlrc, DcpRc, Drude, DrudePole, Dm2,
                        Dielectric, Dummy, Lorentz, LorentzPole, Upml)
-from .source import (Bandpass, Continuous, DifferentiatedGaussian, PointSource)
-
-try:  # defined once the TFSF / Gaussian-beam module is present
-    from .source import TotalFieldScatteredField, GaussianBeam  # noqa: F401
-except ImportError:  # pragma: no cover
-    pass
+from .source import (Bandpass, Continuous, DifferentiatedGaussian, GaussianBeam, PointSource,
+                     TotalFieldScatteredField)
 
 __version__ = "0.1.0"
