@@ -302,6 +302,48 @@ def test_discrete_dispersion_converges_to_the_model(kind):
         assert 3.5 < errs[0] / errs[1] < 4.5 and 3.5 < errs[1] / errs[2] < 4.5, errs
 
 
+# DCP-ADE (src/pw_dcp.hh:105-167,200-225), same z = exp(i w dt):
+#   q (z - a1 - a0/z) = a2 (z + 2 + 1/z) E
+#   p (z - b1 - b0/z) = (b2/z + b3 + b4 z) E
+#   E (z - c3 - c2/z) - c1 [sum_D ((1 - a1) - a0/z) q + sum_C ((1 - b1) - b0/z) p] = c0 C z^(1/2)
+# and the continuum model, for the exp(+i w t) convention used here:
+#   eps_inf + sigma/(i w) + sum_D wD^2 / (-w^2 + i gamma w)
+#     + sum_C 2 A W [(W cos phi - G sin phi) - i w sin phi] / (W^2 + G^2 - w^2 + 2 i G w)
+# (the critical-point term is A W [e^(-i phi)/(W - w + i G) + e^(i phi)/(W + w - i G)]).
+
+def _eps_discrete_dcp(mat, dt, w):
+    space = Cartesian((0, 0, 0))
+    space.dt = dt
+    mat.init(space)
+    z = np.exp(1j * w * dt)
+    c0, c1, c2, c3 = mat.c
+    s = 0
+    for a0, a1, a2 in mat.a:
+        s += ((1 - a1) - a0 / z) * a2 * (z + 2 + 1 / z) / (z - a1 - a0 / z)
+    for b0, b1, b2, b3, b4 in mat.b:
+        s += ((1 - b1) - b0 / z) * (b2 / z + b3 + b4 * z) / (z - b1 - b0 / z)
+    return ((z - c3 - c2 / z) - c1 * s) * dt / (c0 * (z - 1))
+
+
+def test_dcp_ade_dispersion_converges_to_the_model():
+    dps = (g.DrudePole(omega=2.0, gamma=0.3),)
+    cps = (g.CriticalPoint(amp=1.2, phi=-0.6, omega=1.3, gamma=0.4),
+           g.CriticalPoint(amp=0.7, phi=0.9, omega=0.8, gamma=0.15))
+    mat = lambda: g.DcpAde(eps_inf=2.1, sigma=0.15, dps=dps, cps=cps)
+
+    def model(w):
+        e = 2.1 + 0.15 / (1j * w)
+        e += sum(p.omega ** 2 / (-w ** 2 + 1j * p.gamma * w) for p in dps)
+        e += sum(2 * c.amp * c.omega * ((c.omega * np.cos(c.phi) - c.gamma * np.sin(c.phi)) - 1j * w * np.sin(c.phi))
+                 / (c.omega ** 2 + c.gamma ** 2 - w ** 2 + 2j * c.gamma * w) for c in cps)
+        return e
+    for w in (0.3, 0.9, 1.7):
+        exact = model(w)
+        errs = [abs(_eps_discrete_dcp(mat(), dt, w) - exact) / abs(exact) for dt in (0.02, 0.01, 0.005)]
+        assert errs[0] < 0.01
+        assert 3.5 < errs[0] / errs[1] < 4.5 and 3.5 < errs[1] / errs[2] < 4.5, errs
+
+
 # ------------------------------------------------- simulation-level KATs
 # run on the oracle here and on the GPU engine (gpu marker)
 
